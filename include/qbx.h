@@ -1,0 +1,127 @@
+/*
+ * qbx.h -- C-ABI of libqbx_hip.so, the MI355X (gfx950) Gaussian-integral and
+ * Fock-build engine that sits under Quiqbox.jl's integral entry points and
+ * runHartreeFock.
+ *
+ * Plain C types only.  Every call is synchronous on return (one HIP stream per
+ * context) unless its name ends in _device, which enqueues on the context's
+ * stream, or on the caller's stream when one is given, and returns
+ * immediately.  Return value: 0 on success, a negative QBX_E* code on failure;
+ * the message is available from qbx_last_error() (per calling thread).
+ * Indices are 0-based.  Square matrices are n_bf*n_bf and symmetric, so
+ * row- and column-major layouts coincide.  The ERI tensor is n_bf^4 with
+ * eri[((i*n+j)*n+k)*n+l] = (ij|kl) in chemists' notation; being 8-fold
+ * symmetric it reads the same in Julia's column-major order.
+ *
+ * Reference interfaces cited as file:line of Quiqbox.jl v0.6.3.
+ */
+#ifndef QBX_H
+#define QBX_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QBX_OK 0
+#define QBX_EINVAL (-1)   /* bad argument (Julia: DomainError / DimensionMismatch) */
+#define QBX_EHIP (-2)     /* HIP runtime failure                                */
+#define QBX_ENOMEM (-3)   /* allocation failure                                 */
+#define QBX_EUNSUP (-4)   /* unsupported basis (angular momentum above the build) */
+#define QBX_EINTERNAL (-5)
+
+#define QBX_PREC_FP64 0   /* every quartet in FP64                              */
+#define QBX_PREC_MIXED 1  /* FP32 primitive arithmetic for quartets whose Schwarz
+                             bound is below the FP64 cut, FP64 for the rest     */
+
+typedef struct qbx_ctx qbx_ctx;
+typedef struct qbx_basis qbx_basis;
+
+/* Context: one GPU, one stream.  No reference counterpart (the reference has no
+   device state); the Julia glue keeps one per process. */
+int  qbx_init(int32_t device, qbx_ctx** out);
+void qbx_finalize(qbx_ctx* ctx);
+
+/* Flattened basis set.  Replaces the walk from MultiOrbitalData
+   (OrbitalBases.jl:439-468) through getOrbCorePointers (Framework.jl:784-817)
+   that every integral pipeline performs in computeOrbDataIntegral
+   (Framework.jl:895-916).
+   prim_cen  3*n_prim  primitive centres, xyz interleaved
+   prim_xpn  n_prim    exponents
+   prim_ang  3*n_prim  Cartesian exponents (i,j,k)
+   bf_ptr    n_bf+1    CSR row pointer of basis functions over primitives
+   bf_prim   bf_ptr[n_bf]  primitive index of each term
+   bf_w      bf_ptr[n_bf]  final contraction weight of each term (coefficient x
+                           normalization, Framework.jl:748-773)
+   All primitives of one basis function must share centre and (i,j,k). */
+int  qbx_basis_create(qbx_ctx* ctx, int32_t n_prim, const double* prim_cen,
+                      const double* prim_xpn, const int32_t* prim_ang,
+                      int32_t n_bf, const int32_t* bf_ptr, const int32_t* bf_prim,
+                      const double* bf_w, qbx_basis** out);
+void qbx_basis_destroy(qbx_basis* basis);
+/* shell / shell-pair counts of a basis (diagnostics) */
+int  qbx_basis_info(const qbx_basis* basis, int32_t* n_bf, int32_t* n_shell,
+                    int32_t* n_pair, int32_t* max_l);
+
+/* overlaps(bs)            Interface.jl:46-54   -> computeOrbDataIntegral(OneBody, Overlap) */
+int  qbx_overlap(qbx_ctx* ctx, const qbx_basis* basis, double* S);
+/* elecKinetics(bs)        Interface.jl:143-153 (kinetic sampler Samplers.jl:81-107) */
+int  qbx_kinetic(qbx_ctx* ctx, const qbx_basis* basis, double* T);
+/* nucAttractions(nuc, coords, bs)  Interface.jl:236-256; charges Z (n_nuc), R 3*n_nuc;
+   test charge -1 as CoulombMultiPointPotential (Samplers.jl:56-61) */
+int  qbx_nuclear(qbx_ctx* ctx, const qbx_basis* basis, int32_t n_nuc, const double* Z,
+                 const double* R, double* V);
+/* coreHamiltonian(nuc, coords, bs)  Interface.jl:288-312 = kinetic + nuclear */
+int  qbx_core_hamiltonian(qbx_ctx* ctx, const qbx_basis* basis, int32_t n_nuc,
+                          const double* Z, const double* R, double* H);
+/* multipoleMoments(center, degree, bs)  Interface.jl:71-105 */
+int  qbx_multipole(qbx_ctx* ctx, const qbx_basis* basis, const double* center,
+                   const int32_t* degree, double* M);
+/* elecRepulsions(bs)  Interface.jl:356-365 -> computeOrbDataIntegral(TwoBody, Coulomb)
+   (Framework.jl:907-916) dense branch (Framework.jl:640-666).  Limited to
+   n_bf <= 256 (the tensor is n_bf^4 doubles). */
+int  qbx_eri_dense(qbx_ctx* ctx, const qbx_basis* basis, double* eri);
+
+/* Fock two-electron part: getGcore (HartreeFock.jl:305-319) for n_set density
+   pairs at once.  For each set s:
+     G[s] = J[DJ[s]] - K[DK[s]],
+     J[D]_{mn} = sum_{ls} D_{sl} (mn|ls),  K[D]_{mn} = sum_{ls} D_{ls} (ml|sn).
+   RHF passes (2D, D), UHF passes (Da+Db, Da) and (Da+Db, Db) (HartreeFock.jl:322-327).
+   The ERIs are evaluated directly (never stored), shell quartets with
+   Q_ab*Q_cd < schwarz_thr skipped (Q = sqrt of the largest diagonal integral of
+   the shell pair).  DJ, DK, G: n_set*n_bf*n_bf host arrays. */
+int  qbx_jk(qbx_ctx* ctx, qbx_basis* basis, int32_t n_set, const double* DJ,
+            const double* DK, double* G, double schwarz_thr, int32_t precision_mode);
+/* Same with device pointers; enqueued on `stream` (a hipStream_t, NULL for the
+   context stream) and returning without synchronizing. */
+int  qbx_jk_device(qbx_ctx* ctx, qbx_basis* basis, int32_t n_set, const double* dDJ,
+                   const double* dDK, double* dG, double schwarz_thr,
+                   int32_t precision_mode, void* stream);
+
+/* Work statistics of qbx_jk at a threshold (for measurement): per ERI class c
+   (order of qbx_class_table) the number of screened shell quartets, contracted
+   Cartesian quartets and primitive Cartesian quartets, plus the flops that the
+   generated kernels perform per primitive shell quartet and per contracted
+   shell quartet.  Arrays have qbx_n_classes() entries; NULL entries skipped. */
+int32_t qbx_n_classes(void);
+int  qbx_class_table(int32_t* l4 /* 4*n_classes */);
+int  qbx_jk_stats(qbx_ctx* ctx, qbx_basis* basis, double schwarz_thr,
+                  double fp64_cut, int64_t* shell_quartets, int64_t* cart_quartets,
+                  int64_t* prim_quartets, int64_t* fp64_shell_quartets);
+/* When enabled, qbx_jk records per-class kernel time (HIP events on the
+   launching stream); qbx_last_timing writes n_classes milliseconds. */
+int  qbx_set_timing(qbx_ctx* ctx, int32_t enable);
+int  qbx_last_timing(qbx_ctx* ctx, float* ms_per_class, float* ms_total);
+
+/* Mixed-precision cut: quartets whose Schwarz bound exceeds `cut` run in FP64
+   under QBX_PREC_MIXED (default 1e-5). */
+int  qbx_set_mixed_cut(qbx_ctx* ctx, double cut);
+
+/* Last error message of the calling thread; returns its full length. */
+int  qbx_last_error(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,281 @@
+#!/usr/bin/env python3
+"""Generate the per-angular-class ERI code for the HIP kernels.
+
+For every class (la lb | lc ld) with la >= lb, lc >= ld and
+(la, lb) >= (lc, ld) this emits a straight-line device function
+
+    template <typename R> struct EriClass<la, lb, lc, ld, R> {
+        static __device__ void contract(...);   // primitive loop + VRR
+        static __device__ void transfer(...);   // HRR, ket then bra
+    };
+
+Scheme (Head-Gordon & Pople, J. Chem. Phys. 89, 5777 (1988), built on the
+Obara-Saika recurrences, J. Chem. Phys. 84, 3963 (1986)):
+  1. per primitive quartet, the Boys values F_m(T) scaled by the prefactor
+     2 pi^{5/2} K_AB K_CD / (p q sqrt(p+q)) seed [00|00]^(m);
+  2. the vertical recurrence builds [e0|f0]^(0) for |e| in [la, la+lb] and
+     |f| in [lc, lc+ld]; only intermediates reachable from those targets are
+     emitted;
+  3. [e0|f0] is summed over the primitive quartets of the shell quartet
+     (contraction happens before any horizontal transfer);
+  4. the horizontal recurrence moves angular momentum to the second centre
+     of the ket, then of the bra: (a,b+1_i| = (a+1_i,b| + (A-B)_i (a,b|.
+
+Cartesian components follow the order used by Quiqbox (Lexicons.jl:17-36):
+for i = L..0, for j = L-i..0: (i, j, L-i-j).
+
+The generator also writes eri_flops.json: arithmetic operations per primitive
+quartet (VRR + Boys seed) and per contracted quartet (HRR) for each class,
+counted from the emitted expressions (add/sub/mul = 1, fma pattern a*b+c
+counted as 2).
+"""
+import itertools
+import json
+import os
+import sys
+
+LMAX = 2
+
+
+def cart(L):
+    return [(i, j, L - i - j) for i in range(L, -1, -1) for j in range(L - i, -1, -1)]
+
+
+def ncart(L):
+    return (L + 1) * (L + 2) // 2
+
+
+def sub1(t, i):
+    l = list(t)
+    l[i] -= 1
+    return tuple(l)
+
+
+def add1(t, i):
+    l = list(t)
+    l[i] += 1
+    return tuple(l)
+
+
+def pick_axis(t):
+    """axis to lower: the largest component, ties to the lowest axis"""
+    best = -1
+    bi = -1
+    for i in range(3):
+        if t[i] > best:
+            best = t[i]
+            bi = i
+    return bi
+
+
+class Emitter:
+    def __init__(self):
+        self.lines = []
+        self.names = {}
+        self.count = 0
+        self.flops = 0
+
+    def fresh(self, prefix):
+        self.count += 1
+        return "%s%d" % (prefix, self.count)
+
+
+AX = "xyz"
+
+
+def gen_vrr(em, targets):
+    """Emit VRR code; returns dict (e, f) -> variable name for m = 0."""
+    memo = {}
+
+    def V(e, f, m):
+        key = (e, f, m)
+        if key in memo:
+            return memo[key]
+        if sum(e) == 0 and sum(f) == 0:
+            memo[key] = "F[%d]" % m
+            return memo[key]
+        terms = []
+        if sum(f) > 0:
+            i = pick_axis(f)
+            f1 = sub1(f, i)
+            a = V(e, f1, m)
+            b = V(e, f1, m + 1)
+            terms.append(("QC%s" % AX[i], a))
+            terms.append(("WQ%s" % AX[i], b))
+            if f1[i] > 0:
+                f2 = sub1(f1, i)
+                c = V(e, f2, m)
+                d = V(e, f2, m + 1)
+                terms.append(("(R)%d * inv2q" % f1[i], "(%s - rho_q * %s)" % (c, d)))
+                em.flops += 2
+            if e[i] > 0:
+                e1 = sub1(e, i)
+                g = V(e1, f1, m + 1)
+                terms.append(("(R)%d * inv2pq" % e[i], g))
+        else:
+            i = pick_axis(e)
+            e1 = sub1(e, i)
+            a = V(e1, f, m)
+            b = V(e1, f, m + 1)
+            terms.append(("PA%s" % AX[i], a))
+            terms.append(("WP%s" % AX[i], b))
+            if e1[i] > 0:
+                e2 = sub1(e1, i)
+                c = V(e2, f, m)
+                d = V(e2, f, m + 1)
+                terms.append(("(R)%d * inv2p" % e1[i], "(%s - rho_p * %s)" % (c, d)))
+                em.flops += 2
+        name = em.fresh("v")
+        expr = " + ".join("%s * %s" % t for t in terms)
+        em.flops += 2 * len(terms) - 1
+        em.lines.append("const R %s = %s;" % (name, expr))
+        memo[key] = name
+        return name
+
+    out = {}
+    for (e, f) in targets:
+        out[(e, f)] = V(e, f, 0)
+    return out
+
+
+def class_list():
+    pcl = [(a, b) for a in range(LMAX + 1) for b in range(a + 1)]
+    order = {c: n for n, c in enumerate(pcl)}
+    res = []
+    for X in pcl:
+        for Y in pcl:
+            if order[X] >= order[Y]:
+                res.append((X[0], X[1], Y[0], Y[1]))
+    return res, pcl
+
+
+def gen_class(la, lb, lc, ld):
+    """Return (code, flops_prim, flops_contracted, n_acc)."""
+    Lt = la + lb + lc + ld
+    es = [e for L in range(la, la + lb + 1) for e in cart(L)]
+    fs = [f for L in range(lc, lc + ld + 1) for f in cart(L)]
+    eidx = {e: n for n, e in enumerate(es)}
+    fidx = {f: n for n, f in enumerate(fs)}
+    nE, nF = len(es), len(fs)
+
+    # ---- contraction step (VRR inside the primitive loops) ----
+    em = Emitter()
+    targets = [(e, f) for e in es for f in fs]
+    vv = gen_vrr(em, targets)
+    acc_lines = []
+    for (e, f) in targets:
+        acc_lines.append("acc[%d] += %s;" % (eidx[e] * nF + fidx[f], vv[(e, f)]))
+    flops_vrr = em.flops + len(targets)
+    vrr_code = "\n".join("        " + l for l in em.lines + acc_lines)
+
+    # ---- HRR: ket (c,d) from f, then bra (a,b) from e ----
+    hm = Emitter()
+    ket_memo = {}
+
+    def K(e, c, d):  # [e | c d) with e an index into es
+        key = (e, c, d)
+        if key in ket_memo:
+            return ket_memo[key]
+        if sum(d) == 0:
+            r = "acc[%d]" % (e * nF + fidx[c])
+            ket_memo[key] = r
+            return r
+        i = pick_axis(d)
+        d1 = sub1(d, i)
+        a = K(e, add1(c, i), d1)
+        b = K(e, c, d1)
+        name = hm.fresh("k")
+        hm.lines.append("const R %s = %s + CD%s * %s;" % (name, a, AX[i], b))
+        hm.flops += 2
+        ket_memo[key] = name
+        return name
+
+    bra_memo = {}
+
+    def B(a, b, c, d):
+        key = (a, b, c, d)
+        if key in bra_memo:
+            return bra_memo[key]
+        if sum(b) == 0:
+            r = K(eidx[a], c, d)
+            bra_memo[key] = r
+            return r
+        i = pick_axis(b)
+        b1 = sub1(b, i)
+        x = B(add1(a, i), b1, c, d)
+        y = B(a, b1, c, d)
+        name = hm.fresh("h")
+        hm.lines.append("const R %s = %s + AB%s * %s;" % (name, x, AX[i], y))
+        hm.flops += 2
+        bra_memo[key] = name
+        return name
+
+    outs = []
+    A_, B_, C_, D_ = cart(la), cart(lb), cart(lc), cart(ld)
+    nB, nC, nD = len(B_), len(C_), len(D_)
+    for ia, a in enumerate(A_):
+        for ib, b in enumerate(B_):
+            for ic, c in enumerate(C_):
+                for idd, d in enumerate(D_):
+                    r = B(a, b, c, d)
+                    outs.append("out[%d] = %s;" % (((ia * nB + ib) * nC + ic) * nD + idd, r))
+    hrr_code = "\n".join("        " + l for l in hm.lines + outs)
+
+    tag = "%d%d%d%d" % (la, lb, lc, ld)
+    code = []
+    code.append("template <typename R> struct EriClass<%d, %d, %d, %d, R> {" % (la, lb, lc, ld))
+    code.append("    static constexpr int NACC = %d;" % (nE * nF))
+    code.append("    static constexpr int LT = %d;" % Lt)
+    code.append("    static __device__ __forceinline__ void vrr(const R* __restrict__ F, R* __restrict__ acc,")
+    code.append("        R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz,")
+    code.append("        R WQx, R WQy, R WQz, R inv2p, R inv2q, R inv2pq, R rho_p, R rho_q) {")
+    code.append(vrr_code)
+    code.append("    }")
+    code.append("    static __device__ __forceinline__ void hrr(const R* __restrict__ acc, R* __restrict__ out,")
+    code.append("        R ABx, R ABy, R ABz, R CDx, R CDy, R CDz) {")
+    code.append(hrr_code)
+    code.append("    }")
+    code.append("};")
+    return "\n".join(code), flops_vrr, hm.flops, nE * nF
+
+
+def main(outdir):
+    os.makedirs(outdir, exist_ok=True)
+    classes, pcl = class_list()
+    flops = {}
+    for (la, lb, lc, ld) in classes:
+        code, fv, fh, nacc = gen_class(la, lb, lc, ld)
+        tag = "%d%d%d%d" % (la, lb, lc, ld)
+        variants = [("double", "MODE_JK", "jk64"), ("float", "MODE_JK", "jk32"), ("double", "MODE_DENSE", "dense")]
+        if (la, lb) == (lc, ld):
+            variants.append(("double", "MODE_SCHWARZ", "schwarz"))
+        for R, mode, vtag in variants:
+            tu = ["// Generated by gen_eri.py -- do not edit.",
+                  "// ERI class (%d %d | %d %d), %s %s: recurrence code and kernel instantiation."
+                  % (la, lb, lc, ld, R, mode),
+                  '#include "qbx_eri.h"', "", code, "", "namespace qbx {",
+                  "template void launch_eri<%d, %d, %d, %d, %s, %s>(const EriArgs&, hipStream_t);"
+                  % (la, lb, lc, ld, R, mode),
+                  "}  // namespace qbx"]
+            with open(os.path.join(outdir, "eri_%s_%s.hip" % (tag, vtag)), "w") as f:
+                f.write("\n".join(tu) + "\n")
+    # class table for the host dispatcher
+    lines = ["// Generated by gen_eri.py -- do not edit.", "#pragma once",
+             "#define QBX_GEN_LMAX %d" % LMAX,
+             "#define QBX_N_PAIR_CLASSES %d" % len(pcl),
+             "#define QBX_N_CLASSES %d" % len(classes),
+             "static const int qbx_pair_class_l[%d][2] = {%s};" % (
+                 len(pcl), ", ".join("{%d, %d}" % c for c in pcl)),
+             "static const int qbx_class_l[%d][4] = {%s};" % (
+                 len(classes), ", ".join("{%d, %d, %d, %d}" % c for c in classes)),
+             "#define QBX_FOR_EACH_CLASS(X) \\"]
+    lines += ["    X(%d, %d, %d, %d) \\" % c for c in classes]
+    lines.append("")
+    with open(os.path.join(outdir, "eri_table.inc"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with open(os.path.join(outdir, "eri_flops.json"), "w") as f:
+        json.dump(flops, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(os.path.dirname(__file__), "gen"))

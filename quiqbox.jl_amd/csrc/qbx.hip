@@ -1,0 +1,1102 @@
+// qbx.hip -- libqbx_hip.so: Gaussian integrals and direct Fock build for
+// MI355X (gfx950), behind the C-ABI of include/qbx.h.
+//
+// Layout in HBM (one basis):
+//   DevShell[n_shell]            centre, L, component -> basis function map
+//   PrimPair[n_primpair]         primitive-pair records of every shell pair
+//   per pair class (la >= lb):   oriented shell pairs sorted by Schwarz bound
+//                                (descending): shell ids, primitive-pair
+//                                range, Q, A-B
+//   per ERI class and precision: int64 prefix over bra pairs of the number of
+//                                surviving kets, and the first ket of each bra
+//   D (2*n_set*N^2), J, K, G (n_set*N^2) FP64 work buffers
+//
+// One thread evaluates one contracted shell quartet: primitive loop with the
+// generated vertical recurrence, contraction, generated horizontal recurrence,
+// then the J/K digest with per-thread pre-reduction and FP64 atomics.
+#include "qbx.h"
+#include "qbx_device.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "qbx_common.h"
+#include "qbx_eri_api.h"
+#include "gen/eri_table.inc"
+
+using namespace qbx;
+
+namespace {
+using namespace qbx;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+thread_local std::string g_last_error;
+using qbx::QbxError;
+
+template <typename F> int guarded(F&& f)
+{
+    try {
+        f();
+        return QBX_OK;
+    } catch (const QbxError& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return QBX_ENOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return QBX_EINTERNAL;
+    }
+}
+
+template <typename T> struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void ensure(size_t cnt)
+    {
+        if (cnt <= n && p) return;
+        release();
+        size_t c = cnt ? cnt : 1;
+        hipError_t e = hipMalloc((void**)&p, c * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw QbxError(QBX_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+        n = c;
+    }
+    void upload(const std::vector<T>& v, hipStream_t s)
+    {
+        ensure(v.size());
+        if (!v.empty()) QBX_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+constexpr double kPi = 3.14159265358979323846;
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context and basis objects
+// ---------------------------------------------------------------------------
+struct qbx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf<double> boys, bexp;
+    bool timing = false;
+    std::vector<float> last_ms;
+    float last_total = 0.f;
+    double mixed_cut = 1e-5;
+    std::mutex mu;
+};
+
+namespace {
+struct PairClass {
+    // host copies (sorted by Q once Schwarz bounds exist)
+    std::vector<int> shA, shB, ppo, ppn;
+    std::vector<double> Q, AB;
+    DevBuf<int> d_shA, d_shB, d_ppo, d_ppn;
+    DevBuf<double> d_Q, d_AB;
+    int n() const { return (int)shA.size(); }
+};
+
+struct TaskSet {
+    DevBuf<int64_t> prefix;
+    DevBuf<int> koff;
+    int64_t total = 0;
+    int nbra = 0;
+};
+
+struct HostShell {
+    double c[3];
+    int L, nprim, poff;
+    int bf[QBX_MAXC];
+    double sc[QBX_MAXC];
+};
+}  // namespace
+
+struct qbx_basis {
+    qbx_ctx* ctx = nullptr;
+    int nbf = 0, maxL = 0;
+    std::vector<HostShell> shells;
+    std::vector<double> pexp, pcoef;
+    std::vector<PrimPair> pp;
+    PairClass pc[QBX_N_PAIR_CLASSES];
+    DevBuf<PrimPair> d_pp;
+    DevBuf<DevShell> d_sh;
+    // all shell pairs for the one-electron kernels (index into pc lists)
+    std::vector<int2> p1;       // (class, index)
+    DevBuf<int> d_p1A, d_p1B, d_p1o, d_p1n;
+    bool schwarz = false;
+    // task cache
+    bool tasks_valid = false;
+    double task_thr = -1.0, task_cut = -1.0;
+    int task_mode = -1;
+    TaskSet t64[QBX_N_CLASSES], t32[QBX_N_CLASSES];
+    // work buffers
+    DevBuf<double> d_D, d_J, d_K, d_G;
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Boys table (host): F_m(T_i) by the convergent series
+//   F_m(T) = e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1))
+// at the top order, then the stable downward recurrence.
+// ---------------------------------------------------------------------------
+void build_boys_table(std::vector<double>& tab, std::vector<double>& et)
+{
+    tab.assign((size_t)QBX_BOYS_ROWS * QBX_BOYS_NCOL, 0.0);
+    et.assign(QBX_BOYS_ROWS, 0.0);
+    const int mtop = QBX_BOYS_NCOL - 1;
+    for (int i = 0; i < QBX_BOYS_ROWS; ++i) {
+        long double T = (long double)i * QBX_BOYS_DT;
+        long double e = expl(-T);
+        long double term = 1.0L / (2 * mtop + 1), sum = term;
+        for (int k = 1; k < 5000; ++k) {
+            term *= 2.0L * T / (2 * mtop + 2 * k + 1);
+            sum += term;
+            if (term < sum * 1e-21L) break;
+        }
+        long double F[QBX_BOYS_NCOL];
+        F[mtop] = e * sum;
+        for (int m = mtop - 1; m >= 0; --m) F[m] = (2.0L * T * F[m + 1] + e) / (2 * m + 1);
+        for (int m = 0; m < QBX_BOYS_NCOL; ++m) tab[(size_t)i * QBX_BOYS_NCOL + m] = (double)F[m];
+        et[i] = (double)e;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kernels: one-electron integrals
+// ---------------------------------------------------------------------------
+enum OneE { ONE_OVERLAP = 0, ONE_KINETIC = 1, ONE_NUCLEAR = 2, ONE_MULTIPOLE = 3 };
+
+struct OneArgs {
+    const DevShell* sh;
+    const PrimPair* pp;
+    const int* pA;
+    const int* pB;
+    const int* po;
+    const int* pn;
+    int npair;
+    int N;
+    const double* boys;
+    const double* bexp;
+    int nnuc;
+    const double* Z;
+    const double* R;
+    double M[3];
+    int deg[3];
+    double* out;
+};
+
+// Nuclear-attraction primitive value for one component pair and one point C:
+// the Obara-Saika one-body scheme axis by axis with the Boys auxiliary order
+// carried from axis to axis (the scheme of GaussianOrbitals.jl:478-522).
+__device__ double coulomb1_comp(const int* la, const int* lb, const double* PA, const double* PC,
+                                const double* AB, double inv2p, const double* F, int Ltot)
+{
+    double vals[2 * QBX_MAXL + 1];
+    for (int m = 0; m <= Ltot; ++m) vals[m] = F[m];
+    int nUp = Ltot;
+    for (int t = 0; t < 3; ++t) {
+        const int a = la[t], b = lb[t], S = a + b;
+        if (S == 0) continue;
+        double T[2 * QBX_MAXL + 1][2 * QBX_MAXL + 1];
+        for (int j = 0; j <= nUp; ++j) T[0][j] = vals[j];
+        for (int i = 0; i < S; ++i)
+            for (int j = 0; j <= nUp - i - 1; ++j) {
+                double v = PA[t] * T[i][j] - PC[t] * T[i][j + 1];
+                if (i > 0) v += (i * inv2p) * (T[i - 1][j] - T[i - 1][j + 1]);
+                T[i + 1][j] = v;
+            }
+        for (int m = 0; m <= nUp - S; ++m) {
+            double h[2 * QBX_MAXL + 1];
+            for (int i = a; i <= S; ++i) h[i] = T[i][m];
+            for (int s = 0; s < b; ++s)
+                for (int k = a; k < a + b - s; ++k) h[k] = h[k + 1] + AB[t] * h[k];
+            vals[m] = h[a];
+        }
+        nUp -= S;
+    }
+    return vals[0];
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void one_e_kernel(OneArgs a)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int pair = gid / (QBX_MAXC * QBX_MAXC);
+    const int slot = gid % (QBX_MAXC * QBX_MAXC);
+    if (pair >= a.npair) return;
+    const DevShell A = a.sh[a.pA[pair]];
+    const DevShell B = a.sh[a.pB[pair]];
+    const int ca = slot / QBX_MAXC, cb = slot % QBX_MAXC;
+    if (ca >= qbx_ncart(A.L) || cb >= qbx_ncart(B.L)) return;
+    const int ia = A.bf[ca], ib = B.bf[cb];
+    if (ia < 0 || ib < 0) return;
+    int la[3], lb[3];
+    qbx_cart(A.L, ca, la);
+    qbx_cart(B.L, cb, lb);
+    const double AB[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
+    double acc = 0.0;
+    for (int u = 0; u < a.pn[pair]; ++u) {
+        const PrimPair P = a.pp[a.po[pair] + u];
+        const double PA[3] = {P.PAx, P.PAy, P.PAz};
+        const double PB[3] = {P.PAx + AB[0], P.PAy + AB[1], P.PAz + AB[2]};
+        if (KIND == ONE_NUCLEAR) {
+            const int Lt = A.L + B.L;
+            const double Pc[3] = {P.Px, P.Py, P.Pz};
+            double v = 0.0;
+            for (int n = 0; n < a.nnuc; ++n) {
+                const double PC[3] = {Pc[0] - a.R[3 * n], Pc[1] - a.R[3 * n + 1], Pc[2] - a.R[3 * n + 2]};
+                const double T = P.p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+                double F[2 * QBX_MAXL + 1];
+                qbx_boys_rt(Lt, T, F, a.boys, a.bexp);
+                v += a.Z[n] * coulomb1_comp(la, lb, PA, PC, AB, P.inv2p, F, Lt);
+            }
+            acc += -2.0 * kPi * P.Kp * v;
+        } else {
+            double S[3][QBX_OV_MAX][QBX_OV_MAX];
+            const int extra = KIND == ONE_KINETIC ? 2 : (KIND == ONE_MULTIPOLE ? 0 : 0);
+            double v;
+            if (KIND == ONE_MULTIPOLE) {
+                for (int t = 0; t < 3; ++t) qbx_ov1d(PA[t], PB[t], P.inv2p, la[t], lb[t] + a.deg[t], S[t]);
+                v = 1.0;
+                const double BM[3] = {B.x - a.M[0], B.y - a.M[1], B.z - a.M[2]};
+                for (int t = 0; t < 3; ++t) {
+                    const int n = a.deg[t];
+                    double s = 0.0, binom = 1.0;
+                    for (int k = 0; k <= n; ++k) {
+                        double pw = 1.0;
+                        for (int q = 0; q < n - k; ++q) pw *= BM[t];
+                        s += binom * pw * S[t][la[t]][lb[t] + k];
+                        binom = binom * (n - k) / (k + 1);
+                    }
+                    v *= s;
+                }
+            } else {
+                for (int t = 0; t < 3; ++t) qbx_ov1d(PA[t], PB[t], P.inv2p, la[t], lb[t] + extra, S[t]);
+                const double sx = S[0][la[0]][lb[0]], sy = S[1][la[1]][lb[1]], sz = S[2][la[2]][lb[2]];
+                if (KIND == ONE_OVERLAP) {
+                    v = sx * sy * sz;
+                } else {
+                    const double b = P.pad;
+                    double d2[3];
+                    for (int t = 0; t < 3; ++t) {
+                        const int j = lb[t];
+                        double r = -2.0 * b * (2 * j + 1) * S[t][la[t]][j] + 4.0 * b * b * S[t][la[t]][j + 2];
+                        if (j > 1) r += (double)(j * (j - 1)) * S[t][la[t]][j - 2];
+                        d2[t] = r;
+                    }
+                    v = -0.5 * (d2[0] * sy * sz + sx * d2[1] * sz + sx * sy * d2[2]);
+                }
+            }
+            // (pi/p)^{3/2} c_a c_b K_AB = pi^{3/2} Kp / sqrt(p)
+            acc += 5.568327996831707845 * P.Kp * rsqrt(P.p) * v;
+        }
+    }
+    acc *= A.sc[ca] * B.sc[cb];
+    a.out[(size_t)ia * a.N + ib] = acc;
+    a.out[(size_t)ib * a.N + ia] = acc;
+}
+
+__global__ void fill_zero_kernel(double* p, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0.0;
+}
+
+// G[s] = (J + J^T)/2 - (K + K^T)/2
+__global__ void symmetrize_kernel(const double* J, const double* K, double* G, int N, int nset)
+{
+    const size_t NN = (size_t)N * N;
+    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= NN * nset) return;
+    const size_t s = g / NN, r = g % NN;
+    const size_t i = r / N, j = r % N;
+    const double* Js = J + s * NN;
+    const double* Ks = K + s * NN;
+    G[g] = 0.5 * (Js[i * N + j] + Js[j * N + i]) - 0.5 * (Ks[i * N + j] + Ks[j * N + i]);
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+void zero_async(double* p, size_t n, hipStream_t s)
+{
+    if (n == 0) return;
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks), dim3(256), 0, s, p, n);
+    QBX_HIP(hipGetLastError());
+}
+
+int pair_class_index(int la, int lb)
+{
+    for (int c = 0; c < QBX_N_PAIR_CLASSES; ++c)
+        if (qbx_pair_class_l[c][0] == la && qbx_pair_class_l[c][1] == lb) return c;
+    return -1;
+}
+
+int class_pc(int cls, int side)
+{
+    return pair_class_index(qbx_class_l[cls][2 * side], qbx_class_l[cls][2 * side + 1]);
+}
+
+PairClassDev pcdev(const PairClass& c)
+{
+    PairClassDev d;
+    d.n = c.n();
+    d.shA = c.d_shA.p;
+    d.shB = c.d_shB.p;
+    d.ppo = c.d_ppo.p;
+    d.ppn = c.d_ppn.p;
+    d.AB = c.d_AB.p;
+    return d;
+}
+
+#define QBX_TAB_ENTRY(R, MODE)
+#define QBX_E64JK(a, b, c, d) &launch_eri<a, b, c, d, double, MODE_JK>,
+#define QBX_E32JK(a, b, c, d) &launch_eri<a, b, c, d, float, MODE_JK>,
+#define QBX_EDENSE(a, b, c, d) &launch_eri<a, b, c, d, double, MODE_DENSE>,
+const EriLaunch k_jk64[QBX_N_CLASSES] = {QBX_FOR_EACH_CLASS(QBX_E64JK)};
+const EriLaunch k_jk32[QBX_N_CLASSES] = {QBX_FOR_EACH_CLASS(QBX_E32JK)};
+const EriLaunch k_dense[QBX_N_CLASSES] = {QBX_FOR_EACH_CLASS(QBX_EDENSE)};
+
+EriLaunch schwarz_launcher(int la, int lb)
+{
+#define QBX_SW(a, b) if (la == a && lb == b) return &launch_eri<a, b, a, b, double, MODE_SCHWARZ>;
+    QBX_SW(0, 0) QBX_SW(1, 0) QBX_SW(1, 1) QBX_SW(2, 0) QBX_SW(2, 1) QBX_SW(2, 2)
+#undef QBX_SW
+    return nullptr;
+}
+
+EriArgs base_args(qbx_basis* b, int cls)
+{
+    EriArgs a;
+    std::memset(&a, 0, sizeof(a));
+    const int X = class_pc(cls, 0), Y = class_pc(cls, 1);
+    a.bra = pcdev(b->pc[X]);
+    a.ket = pcdev(b->pc[Y]);
+    a.pp = b->d_pp.p;
+    a.sh = b->d_sh.p;
+    a.boys = b->ctx->boys.p;
+    a.bexp = b->ctx->bexp.p;
+    a.same = X == Y;
+    a.N = b->nbf;
+    return a;
+}
+
+// Schwarz bounds, then every pair class sorted by descending bound.
+void ensure_schwarz(qbx_basis* b)
+{
+    if (b->schwarz) return;
+    hipStream_t st = b->ctx->stream;
+    std::vector<DevBuf<double>> q(QBX_N_PAIR_CLASSES);
+    for (int c = 0; c < QBX_N_PAIR_CLASSES; ++c) {
+        PairClass& P = b->pc[c];
+        if (P.n() == 0) continue;
+        q[c].ensure(P.n());
+        EriArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.bra = pcdev(P);
+        a.ket = pcdev(P);
+        a.pp = b->d_pp.p;
+        a.sh = b->d_sh.p;
+        a.boys = b->ctx->boys.p;
+        a.bexp = b->ctx->bexp.p;
+        a.total = P.n();
+        a.nbra = P.n();
+        a.same = 1;
+        a.N = b->nbf;
+        a.qout = q[c].p;
+        EriLaunch f = schwarz_launcher(qbx_pair_class_l[c][0], qbx_pair_class_l[c][1]);
+        f(a, st);
+    }
+    for (int c = 0; c < QBX_N_PAIR_CLASSES; ++c) {
+        PairClass& P = b->pc[c];
+        const int n = P.n();
+        if (n == 0) continue;
+        std::vector<double> Q(n);
+        QBX_HIP(hipMemcpyAsync(Q.data(), q[c].p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        QBX_HIP(hipStreamSynchronize(st));
+        std::vector<int> perm(n);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return Q[x] > Q[y]; });
+        PairClass S;
+        for (int k = 0; k < n; ++k) {
+            const int o = perm[k];
+            S.shA.push_back(P.shA[o]);
+            S.shB.push_back(P.shB[o]);
+            S.ppo.push_back(P.ppo[o]);
+            S.ppn.push_back(P.ppn[o]);
+            S.Q.push_back(Q[o]);
+            for (int t = 0; t < 3; ++t) S.AB.push_back(P.AB[3 * o + t]);
+        }
+        P.shA.swap(S.shA);
+        P.shB.swap(S.shB);
+        P.ppo.swap(S.ppo);
+        P.ppn.swap(S.ppn);
+        P.Q.swap(S.Q);
+        P.AB.swap(S.AB);
+        P.d_shA.upload(P.shA, st);
+        P.d_shB.upload(P.shB, st);
+        P.d_ppo.upload(P.ppo, st);
+        P.d_ppn.upload(P.ppn, st);
+        P.d_AB.upload(P.AB, st);
+        P.d_Q.upload(P.Q, st);
+    }
+    QBX_HIP(hipStreamSynchronize(st));
+    b->schwarz = true;
+}
+
+// number of leading entries of a descending array that are >= v
+int count_ge(const std::vector<double>& q, double v)
+{
+    return (int)(std::upper_bound(q.begin(), q.end(), v, [](double x, double y) { return x > y; }) - q.begin());
+}
+
+// Screened task lists: for bra i of class X, kets j of class Y with
+// Q_i Q_j >= thr (and j <= i when X == Y).  Under mixed precision the
+// surviving kets split at Q_i Q_j >= cut into an FP64 head and an FP32 tail.
+struct TaskCounts {
+    std::vector<int64_t> p64, p32;
+    std::vector<int> k64, k32;
+};
+
+void task_counts(const qbx_basis* b, int cls, double thr, int mode, double cut, TaskCounts& tc)
+{
+    const int X = class_pc(cls, 0), Y = class_pc(cls, 1);
+    const PairClass& PX = b->pc[X];
+    const PairClass& PY = b->pc[Y];
+    const int nx = PX.n();
+    tc.p64.assign(nx + 1, 0);
+    tc.p32.assign(nx + 1, 0);
+    tc.k64.assign(std::max(nx, 1), 0);
+    tc.k32.assign(std::max(nx, 1), 0);
+    for (int i = 0; i < nx; ++i) {
+        const double qi = PX.Q[i];
+        int cnt;
+        if (thr <= 0.0) cnt = PY.n();
+        else cnt = qi > 0.0 ? count_ge(PY.Q, thr / qi) : 0;
+        if (X == Y) cnt = std::min(cnt, i + 1);
+        int c64 = cnt;
+        if (mode == QBX_PREC_MIXED) {
+            int kc = qi > 0.0 ? count_ge(PY.Q, cut / qi) : 0;
+            c64 = std::min(cnt, kc);
+        }
+        tc.k64[i] = 0;
+        tc.k32[i] = c64;
+        tc.p64[i + 1] = tc.p64[i] + c64;
+        tc.p32[i + 1] = tc.p32[i] + (cnt - c64);
+    }
+}
+
+void ensure_tasks(qbx_basis* b, double thr, int mode, double cut)
+{
+    if (mode != QBX_PREC_MIXED) cut = 0.0;
+    if (b->tasks_valid && b->task_thr == thr && b->task_mode == mode && b->task_cut == cut) return;
+    hipStream_t st = b->ctx->stream;
+    for (int cls = 0; cls < QBX_N_CLASSES; ++cls) {
+        TaskCounts tc;
+        task_counts(b, cls, thr, mode, cut, tc);
+        const int nx = b->pc[class_pc(cls, 0)].n();
+        b->t64[cls].prefix.upload(tc.p64, st);
+        b->t64[cls].koff.upload(tc.k64, st);
+        b->t64[cls].total = tc.p64[nx];
+        b->t64[cls].nbra = nx;
+        b->t32[cls].prefix.upload(tc.p32, st);
+        b->t32[cls].koff.upload(tc.k32, st);
+        b->t32[cls].total = tc.p32[nx];
+        b->t32[cls].nbra = nx;
+    }
+    QBX_HIP(hipStreamSynchronize(st));
+    b->tasks_valid = true;
+    b->task_thr = thr;
+    b->task_mode = mode;
+    b->task_cut = cut;
+}
+
+void check_ctx_basis(qbx_ctx* ctx, const qbx_basis* b)
+{
+    if (!ctx) throw QbxError(QBX_EINVAL, "null context");
+    if (!b) throw QbxError(QBX_EINVAL, "null basis");
+    if (b->ctx != ctx) throw QbxError(QBX_EINVAL, "basis belongs to another context");
+}
+
+void run_one_e(qbx_ctx* ctx, const qbx_basis* b, int kind, int nnuc, const double* Z, const double* R,
+               const double* M, const int32_t* deg, double* out)
+{
+    check_ctx_basis(ctx, b);
+    if (!out) throw QbxError(QBX_EINVAL, "null output");
+    QBX_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t NN = (size_t)b->nbf * b->nbf;
+    DevBuf<double> dout, dZ, dR;
+    dout.ensure(NN);
+    zero_async(dout.p, NN, st);
+    OneArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.sh = b->d_sh.p;
+    a.pp = b->d_pp.p;
+    a.pA = b->d_p1A.p;
+    a.pB = b->d_p1B.p;
+    a.po = b->d_p1o.p;
+    a.pn = b->d_p1n.p;
+    a.npair = (int)b->p1.size();
+    a.N = b->nbf;
+    a.boys = ctx->boys.p;
+    a.bexp = ctx->bexp.p;
+    a.out = dout.p;
+    if (kind == ONE_NUCLEAR) {
+        if (nnuc <= 0 || !Z || !R) throw QbxError(QBX_EINVAL, "nuclear attraction needs at least one nucleus");
+        std::vector<double> z(Z, Z + nnuc), r(R, R + 3 * nnuc);
+        dZ.upload(z, st);
+        dR.upload(r, st);
+        a.nnuc = nnuc;
+        a.Z = dZ.p;
+        a.R = dR.p;
+    }
+    if (kind == ONE_MULTIPOLE) {
+        for (int t = 0; t < 3; ++t) {
+            if (deg[t] < 0 || deg[t] > 4) throw QbxError(QBX_EUNSUP, "multipole degree per axis must be in [0, 4]");
+            a.M[t] = M[t];
+            a.deg[t] = deg[t];
+        }
+    }
+    const int64_t threads = (int64_t)a.npair * QBX_MAXC * QBX_MAXC;
+    if (threads > 0) {
+        const int blocks = (int)((threads + 255) / 256);
+        switch (kind) {
+        case ONE_OVERLAP: hipLaunchKernelGGL(one_e_kernel<ONE_OVERLAP>, dim3(blocks), dim3(256), 0, st, a); break;
+        case ONE_KINETIC: hipLaunchKernelGGL(one_e_kernel<ONE_KINETIC>, dim3(blocks), dim3(256), 0, st, a); break;
+        case ONE_NUCLEAR: hipLaunchKernelGGL(one_e_kernel<ONE_NUCLEAR>, dim3(blocks), dim3(256), 0, st, a); break;
+        default: hipLaunchKernelGGL(one_e_kernel<ONE_MULTIPOLE>, dim3(blocks), dim3(256), 0, st, a); break;
+        }
+        QBX_HIP(hipGetLastError());
+    }
+    QBX_HIP(hipMemcpyAsync(out, dout.p, NN * sizeof(double), hipMemcpyDeviceToHost, st));
+    QBX_HIP(hipStreamSynchronize(st));
+}
+
+void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* dG, double thr, int mode,
+               hipStream_t st)
+{
+    ensure_schwarz(b);
+    ensure_tasks(b, thr, mode, ctx->mixed_cut);
+    const size_t NN = (size_t)b->nbf * b->nbf;
+    b->d_J.ensure(NN * nset);
+    b->d_K.ensure(NN * nset);
+    zero_async(b->d_J.p, NN * nset, st);
+    zero_async(b->d_K.p, NN * nset, st);
+    std::vector<hipEvent_t> ev;
+    if (ctx->timing) {
+        ev.resize(2 * QBX_N_CLASSES + 2);
+        for (auto& e : ev) QBX_HIP(hipEventCreate(&e));
+        QBX_HIP(hipEventRecord(ev[2 * QBX_N_CLASSES], st));
+    }
+    for (int cls = 0; cls < QBX_N_CLASSES; ++cls) {
+        if (ctx->timing) QBX_HIP(hipEventRecord(ev[2 * cls], st));
+        EriArgs a = base_args(b, cls);
+        a.D = dD;
+        a.J = b->d_J.p;
+        a.K = b->d_K.p;
+        a.nset = nset;
+        const TaskSet& t64 = b->t64[cls];
+        a.prefix = t64.prefix.p;
+        a.koff = t64.koff.p;
+        a.total = t64.total;
+        a.nbra = t64.nbra;
+        k_jk64[cls](a, st);
+        if (mode == QBX_PREC_MIXED) {
+            const TaskSet& t32 = b->t32[cls];
+            a.prefix = t32.prefix.p;
+            a.koff = t32.koff.p;
+            a.total = t32.total;
+            a.nbra = t32.nbra;
+            k_jk32[cls](a, st);
+        }
+        if (ctx->timing) QBX_HIP(hipEventRecord(ev[2 * cls + 1], st));
+    }
+    const size_t tot = NN * nset;
+    hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                       b->d_J.p, b->d_K.p, dG, b->nbf, nset);
+    QBX_HIP(hipGetLastError());
+    if (ctx->timing) {
+        QBX_HIP(hipEventRecord(ev[2 * QBX_N_CLASSES + 1], st));
+        QBX_HIP(hipEventSynchronize(ev[2 * QBX_N_CLASSES + 1]));
+        ctx->last_ms.assign(QBX_N_CLASSES, 0.f);
+        for (int cls = 0; cls < QBX_N_CLASSES; ++cls)
+            QBX_HIP(hipEventElapsedTime(&ctx->last_ms[cls], ev[2 * cls], ev[2 * cls + 1]));
+        QBX_HIP(hipEventElapsedTime(&ctx->last_total, ev[2 * QBX_N_CLASSES], ev[2 * QBX_N_CLASSES + 1]));
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int qbx_last_error(char* buf, size_t len)
+{
+    const std::string& m = g_last_error;
+    if (buf && len > 0) {
+        size_t n = std::min(len - 1, m.size());
+        std::memcpy(buf, m.data(), n);
+        buf[n] = 0;
+    }
+    return (int)m.size();
+}
+
+int qbx_init(int32_t device, qbx_ctx** out)
+{
+    return guarded([&] {
+        if (!out) throw QbxError(QBX_EINVAL, "null output pointer");
+        *out = nullptr;
+        int ndev = 0;
+        QBX_HIP(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev)
+            throw QbxError(QBX_EINVAL, "device " + std::to_string(device) + " out of range (" +
+                                           std::to_string(ndev) + " devices)");
+        std::unique_ptr<qbx_ctx> c(new qbx_ctx());
+        c->device = device;
+        QBX_HIP(hipSetDevice(device));
+        QBX_HIP(hipStreamCreate(&c->stream));
+        std::vector<double> tab, et;
+        build_boys_table(tab, et);
+        c->boys.upload(tab, c->stream);
+        c->bexp.upload(et, c->stream);
+        QBX_HIP(hipStreamSynchronize(c->stream));
+        *out = c.release();
+    });
+}
+
+void qbx_finalize(qbx_ctx* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->boys.release();
+    ctx->bexp.release();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int qbx_basis_create(qbx_ctx* ctx, int32_t n_prim, const double* prim_cen, const double* prim_xpn,
+                     const int32_t* prim_ang, int32_t n_bf, const int32_t* bf_ptr, const int32_t* bf_prim,
+                     const double* bf_w, qbx_basis** out)
+{
+    return guarded([&] {
+        if (!ctx || !out) throw QbxError(QBX_EINVAL, "null context or output pointer");
+        *out = nullptr;
+        if (n_prim <= 0 || n_bf <= 0) throw QbxError(QBX_EINVAL, "empty basis");
+        if (!prim_cen || !prim_xpn || !prim_ang || !bf_ptr || !bf_prim || !bf_w)
+            throw QbxError(QBX_EINVAL, "null basis array");
+        if (bf_ptr[0] != 0) throw QbxError(QBX_EINVAL, "bf_ptr[0] must be 0");
+        std::unique_ptr<qbx_basis> b(new qbx_basis());
+        b->ctx = ctx;
+        b->nbf = n_bf;
+        // ---- group basis functions into shells ----
+        struct Key {
+            double c[3];
+            int L;
+            std::vector<double> x;
+        };
+        std::vector<Key> keys;
+        for (int f = 0; f < n_bf; ++f) {
+            const int s0 = bf_ptr[f], s1 = bf_ptr[f + 1];
+            if (s1 <= s0) throw QbxError(QBX_EINVAL, "basis function " + std::to_string(f) + " has no primitive");
+            const int p0 = bf_prim[s0];
+            if (p0 < 0 || p0 >= n_prim) throw QbxError(QBX_EINVAL, "primitive index out of range");
+            const double* c0 = prim_cen + 3 * p0;
+            const int32_t* a0 = prim_ang + 3 * p0;
+            for (int s = s0; s < s1; ++s) {
+                const int p = bf_prim[s];
+                if (p < 0 || p >= n_prim) throw QbxError(QBX_EINVAL, "primitive index out of range");
+                if (prim_xpn[p] <= 0.0) throw QbxError(QBX_EINVAL, "exponents must be positive");
+                for (int t = 0; t < 3; ++t)
+                    if (prim_cen[3 * p + t] != c0[t] || prim_ang[3 * p + t] != a0[t])
+                        throw QbxError(QBX_EINVAL, "basis function " + std::to_string(f) +
+                                                       " mixes centres or Cartesian exponents");
+            }
+            if (a0[0] < 0 || a0[1] < 0 || a0[2] < 0) throw QbxError(QBX_EINVAL, "negative angular momentum");
+            const int L = a0[0] + a0[1] + a0[2];
+            if (L > QBX_MAXL)
+                throw QbxError(QBX_EUNSUP, "angular momentum " + std::to_string(L) + " above the build limit " +
+                                               std::to_string(QBX_MAXL));
+            b->maxL = std::max(b->maxL, L);
+            int comp = -1;
+            for (int c = 0; c < qbx_ncart(L); ++c) {
+                int ijk[3];
+                qbx_cart(L, c, ijk);
+                if (ijk[0] == a0[0] && ijk[1] == a0[1] && ijk[2] == a0[2]) comp = c;
+            }
+            // try to join an existing shell with identical centre/exponents and
+            // proportional contraction weights
+            int found = -1;
+            double scale = 1.0;
+            for (int sh = (int)b->shells.size() - 1; sh >= 0 && found < 0; --sh) {
+                const HostShell& H = b->shells[sh];
+                const Key& K = keys[sh];
+                if (H.L != L || H.bf[comp] >= 0 || (int)K.x.size() != s1 - s0) continue;
+                if (K.c[0] != c0[0] || K.c[1] != c0[1] || K.c[2] != c0[2]) continue;
+                bool ok = true;
+                for (int s = s0; s < s1 && ok; ++s) ok = prim_xpn[bf_prim[s]] == K.x[s - s0];
+                if (!ok) continue;
+                int kmax = 0;
+                for (int k = 1; k < H.nprim; ++k)
+                    if (std::fabs(b->pcoef[H.poff + k]) > std::fabs(b->pcoef[H.poff + kmax])) kmax = k;
+                const double cref = b->pcoef[H.poff + kmax];
+                if (cref == 0.0) continue;
+                const double sc = bf_w[s0 + kmax] / cref;
+                for (int k = 0; k < H.nprim && ok; ++k) {
+                    const double w = bf_w[s0 + k], pred = sc * b->pcoef[H.poff + k];
+                    ok = std::fabs(w - pred) <= 1e-13 * (std::fabs(w) + std::fabs(pred)) + 1e-300;
+                }
+                if (ok) {
+                    found = sh;
+                    scale = sc;
+                }
+            }
+            if (found < 0) {
+                HostShell H;
+                for (int t = 0; t < 3; ++t) H.c[t] = c0[t];
+                H.L = L;
+                H.nprim = s1 - s0;
+                H.poff = (int)b->pexp.size();
+                for (int c = 0; c < QBX_MAXC; ++c) {
+                    H.bf[c] = -1;
+                    H.sc[c] = 0.0;
+                }
+                Key K;
+                for (int t = 0; t < 3; ++t) K.c[t] = c0[t];
+                K.L = L;
+                for (int s = s0; s < s1; ++s) {
+                    b->pexp.push_back(prim_xpn[bf_prim[s]]);
+                    b->pcoef.push_back(bf_w[s]);
+                    K.x.push_back(prim_xpn[bf_prim[s]]);
+                }
+                b->shells.push_back(H);
+                keys.push_back(K);
+                found = (int)b->shells.size() - 1;
+                scale = 1.0;
+            }
+            b->shells[found].bf[comp] = f;
+            b->shells[found].sc[comp] = scale;
+        }
+        // ---- shell pairs, oriented so that L_A >= L_B ----
+        const int ns = (int)b->shells.size();
+        for (int s1 = 0; s1 < ns; ++s1)
+            for (int s2 = 0; s2 <= s1; ++s2) {
+                int A = s1, B = s2;
+                if (b->shells[A].L < b->shells[B].L) std::swap(A, B);
+                const HostShell& HA = b->shells[A];
+                const HostShell& HB = b->shells[B];
+                const int cl = pair_class_index(HA.L, HB.L);
+                PairClass& P = b->pc[cl];
+                const double AB[3] = {HA.c[0] - HB.c[0], HA.c[1] - HB.c[1], HA.c[2] - HB.c[2]};
+                const double r2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+                P.shA.push_back(A);
+                P.shB.push_back(B);
+                P.ppo.push_back((int)b->pp.size());
+                for (int u = 0; u < HA.nprim; ++u)
+                    for (int v = 0; v < HB.nprim; ++v) {
+                        const double a = b->pexp[HA.poff + u], bb = b->pexp[HB.poff + v];
+                        const double p = a + bb;
+                        PrimPair q;
+                        q.p = p;
+                        q.inv2p = 0.5 / p;
+                        q.Px = (a * HA.c[0] + bb * HB.c[0]) / p;
+                        q.Py = (a * HA.c[1] + bb * HB.c[1]) / p;
+                        q.Pz = (a * HA.c[2] + bb * HB.c[2]) / p;
+                        q.PAx = q.Px - HA.c[0];
+                        q.PAy = q.Py - HA.c[1];
+                        q.PAz = q.Pz - HA.c[2];
+                        q.Kp = b->pcoef[HA.poff + u] * b->pcoef[HB.poff + v] * std::exp(-a * bb / p * r2) / p;
+                        q.pad = bb;
+                        b->pp.push_back(q);
+                    }
+                P.ppn.push_back(HA.nprim * HB.nprim);
+                for (int t = 0; t < 3; ++t) P.AB.push_back(AB[t]);
+                P.Q.push_back(0.0);
+                b->p1.push_back(make_int2(cl, P.n() - 1));
+            }
+        // ---- upload ----
+        QBX_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        std::vector<DevShell> ds(ns);
+        for (int s = 0; s < ns; ++s) {
+            const HostShell& H = b->shells[s];
+            ds[s].x = H.c[0];
+            ds[s].y = H.c[1];
+            ds[s].z = H.c[2];
+            ds[s].L = H.L;
+            ds[s].nprim = H.nprim;
+            for (int c = 0; c < QBX_MAXC; ++c) {
+                ds[s].bf[c] = H.bf[c];
+                ds[s].sc[c] = H.sc[c];
+            }
+        }
+        b->d_sh.upload(ds, st);
+        b->d_pp.upload(b->pp, st);
+        for (int c = 0; c < QBX_N_PAIR_CLASSES; ++c) {
+            PairClass& P = b->pc[c];
+            P.d_shA.upload(P.shA, st);
+            P.d_shB.upload(P.shB, st);
+            P.d_ppo.upload(P.ppo, st);
+            P.d_ppn.upload(P.ppn, st);
+            P.d_AB.upload(P.AB, st);
+            P.d_Q.upload(P.Q, st);
+        }
+        std::vector<int> p1A, p1B, p1o, p1n;
+        for (const int2& e : b->p1) {
+            const PairClass& P = b->pc[e.x];
+            p1A.push_back(P.shA[e.y]);
+            p1B.push_back(P.shB[e.y]);
+            p1o.push_back(P.ppo[e.y]);
+            p1n.push_back(P.ppn[e.y]);
+        }
+        b->d_p1A.upload(p1A, st);
+        b->d_p1B.upload(p1B, st);
+        b->d_p1o.upload(p1o, st);
+        b->d_p1n.upload(p1n, st);
+        QBX_HIP(hipStreamSynchronize(st));
+        *out = b.release();
+    });
+}
+
+void qbx_basis_destroy(qbx_basis* b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+}
+
+int qbx_basis_info(const qbx_basis* b, int32_t* n_bf, int32_t* n_shell, int32_t* n_pair, int32_t* max_l)
+{
+    return guarded([&] {
+        if (!b) throw QbxError(QBX_EINVAL, "null basis");
+        if (n_bf) *n_bf = b->nbf;
+        if (n_shell) *n_shell = (int32_t)b->shells.size();
+        if (n_pair) *n_pair = (int32_t)b->p1.size();
+        if (max_l) *max_l = b->maxL;
+    });
+}
+
+int qbx_overlap(qbx_ctx* ctx, const qbx_basis* b, double* S)
+{
+    return guarded([&] { run_one_e(ctx, b, ONE_OVERLAP, 0, nullptr, nullptr, nullptr, nullptr, S); });
+}
+
+int qbx_kinetic(qbx_ctx* ctx, const qbx_basis* b, double* T)
+{
+    return guarded([&] { run_one_e(ctx, b, ONE_KINETIC, 0, nullptr, nullptr, nullptr, nullptr, T); });
+}
+
+int qbx_nuclear(qbx_ctx* ctx, const qbx_basis* b, int32_t n_nuc, const double* Z, const double* R, double* V)
+{
+    return guarded([&] { run_one_e(ctx, b, ONE_NUCLEAR, n_nuc, Z, R, nullptr, nullptr, V); });
+}
+
+int qbx_core_hamiltonian(qbx_ctx* ctx, const qbx_basis* b, int32_t n_nuc, const double* Z, const double* R,
+                         double* H)
+{
+    return guarded([&] {
+        check_ctx_basis(ctx, b);
+        if (!H) throw QbxError(QBX_EINVAL, "null output");
+        const size_t NN = (size_t)b->nbf * b->nbf;
+        std::vector<double> T(NN);
+        run_one_e(ctx, b, ONE_KINETIC, 0, nullptr, nullptr, nullptr, nullptr, T.data());
+        run_one_e(ctx, b, ONE_NUCLEAR, n_nuc, Z, R, nullptr, nullptr, H);
+        for (size_t i = 0; i < NN; ++i) H[i] = T[i] + H[i];
+    });
+}
+
+int qbx_multipole(qbx_ctx* ctx, const qbx_basis* b, const double* center, const int32_t* degree, double* M)
+{
+    return guarded([&] {
+        if (!center || !degree) throw QbxError(QBX_EINVAL, "null multipole centre or degree");
+        run_one_e(ctx, b, ONE_MULTIPOLE, 0, nullptr, nullptr, center, degree, M);
+    });
+}
+
+int qbx_eri_dense(qbx_ctx* ctx, const qbx_basis* bc, double* eri)
+{
+    return guarded([&] {
+        qbx_basis* b = const_cast<qbx_basis*>(bc);
+        check_ctx_basis(ctx, b);
+        if (!eri) throw QbxError(QBX_EINVAL, "null output");
+        if (b->nbf > 256) throw QbxError(QBX_EINVAL, "dense ERI tensor limited to 256 basis functions");
+        QBX_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        ensure_schwarz(b);
+        const size_t N = b->nbf, N4 = N * N * N * N;
+        DevBuf<double> d;
+        d.ensure(N4);
+        zero_async(d.p, N4, st);
+        for (int cls = 0; cls < QBX_N_CLASSES; ++cls) {
+            TaskCounts tc;
+            task_counts(b, cls, 0.0, QBX_PREC_FP64, 0.0, tc);
+            const int nx = b->pc[class_pc(cls, 0)].n();
+            if (tc.p64[nx] == 0) continue;
+            DevBuf<int64_t> pre;
+            DevBuf<int> ko;
+            pre.upload(tc.p64, st);
+            ko.upload(tc.k64, st);
+            EriArgs a = base_args(b, cls);
+            a.prefix = pre.p;
+            a.koff = ko.p;
+            a.total = tc.p64[nx];
+            a.nbra = nx;
+            a.eri = d.p;
+            k_dense[cls](a, st);
+            QBX_HIP(hipStreamSynchronize(st));
+        }
+        QBX_HIP(hipMemcpyAsync(eri, d.p, N4 * sizeof(double), hipMemcpyDeviceToHost, st));
+        QBX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int qbx_jk_device(qbx_ctx* ctx, qbx_basis* b, int32_t n_set, const double* dDJ, const double* dDK, double* dG,
+                  double thr, int32_t mode, void* stream)
+{
+    return guarded([&] {
+        check_ctx_basis(ctx, b);
+        if (n_set <= 0) throw QbxError(QBX_EINVAL, "n_set must be positive");
+        if (!dDJ || !dDK || !dG) throw QbxError(QBX_EINVAL, "null density or output pointer");
+        if (mode != QBX_PREC_FP64 && mode != QBX_PREC_MIXED) throw QbxError(QBX_EINVAL, "unknown precision mode");
+        if (!(thr >= 0.0)) throw QbxError(QBX_EINVAL, "Schwarz threshold must be >= 0");
+        QBX_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+        const size_t NN = (size_t)b->nbf * b->nbf;
+        // the kernels read DJ and DK from one contiguous buffer
+        b->d_D.ensure(2 * NN * n_set);
+        QBX_HIP(hipMemcpyAsync(b->d_D.p, dDJ, NN * n_set * sizeof(double), hipMemcpyDeviceToDevice, st));
+        QBX_HIP(hipMemcpyAsync(b->d_D.p + NN * n_set, dDK, NN * n_set * sizeof(double), hipMemcpyDeviceToDevice, st));
+        jk_device(ctx, b, n_set, b->d_D.p, dG, thr, mode, st);
+    });
+}
+
+int qbx_jk(qbx_ctx* ctx, qbx_basis* b, int32_t n_set, const double* DJ, const double* DK, double* G, double thr,
+           int32_t mode)
+{
+    return guarded([&] {
+        check_ctx_basis(ctx, b);
+        if (n_set <= 0) throw QbxError(QBX_EINVAL, "n_set must be positive");
+        if (!DJ || !DK || !G) throw QbxError(QBX_EINVAL, "null density or output pointer");
+        if (mode != QBX_PREC_FP64 && mode != QBX_PREC_MIXED) throw QbxError(QBX_EINVAL, "unknown precision mode");
+        if (!(thr >= 0.0)) throw QbxError(QBX_EINVAL, "Schwarz threshold must be >= 0");
+        QBX_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        const size_t NN = (size_t)b->nbf * b->nbf;
+        b->d_D.ensure(2 * NN * n_set);
+        b->d_G.ensure(NN * n_set);
+        QBX_HIP(hipMemcpyAsync(b->d_D.p, DJ, NN * n_set * sizeof(double), hipMemcpyHostToDevice, st));
+        QBX_HIP(hipMemcpyAsync(b->d_D.p + NN * n_set, DK, NN * n_set * sizeof(double), hipMemcpyHostToDevice, st));
+        jk_device(ctx, b, n_set, b->d_D.p, b->d_G.p, thr, mode, st);
+        QBX_HIP(hipMemcpyAsync(G, b->d_G.p, NN * n_set * sizeof(double), hipMemcpyDeviceToHost, st));
+        QBX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int32_t qbx_n_classes(void) { return QBX_N_CLASSES; }
+
+int qbx_class_table(int32_t* l4)
+{
+    return guarded([&] {
+        if (!l4) throw QbxError(QBX_EINVAL, "null output");
+        for (int c = 0; c < QBX_N_CLASSES; ++c)
+            for (int t = 0; t < 4; ++t) l4[4 * c + t] = qbx_class_l[c][t];
+    });
+}
+
+int qbx_jk_stats(qbx_ctx* ctx, qbx_basis* b, double thr, double cut, int64_t* shq, int64_t* cartq, int64_t* primq,
+                 int64_t* fp64q)
+{
+    return guarded([&] {
+        check_ctx_basis(ctx, b);
+        QBX_HIP(hipSetDevice(ctx->device));
+        ensure_schwarz(b);
+        for (int cls = 0; cls < QBX_N_CLASSES; ++cls) {
+            TaskCounts tc;
+            task_counts(b, cls, thr, cut > 0.0 ? QBX_PREC_MIXED : QBX_PREC_FP64, cut, tc);
+            const PairClass& PX = b->pc[class_pc(cls, 0)];
+            const PairClass& PY = b->pc[class_pc(cls, 1)];
+            const int nx = PX.n();
+            // prefix sums of ket primitive-pair counts in sorted order
+            std::vector<int64_t> kp(PY.n() + 1, 0);
+            for (int j = 0; j < PY.n(); ++j) kp[j + 1] = kp[j] + PY.ppn[j];
+            int64_t sq = 0, pq = 0, s64 = 0;
+            for (int i = 0; i < nx; ++i) {
+                const int64_t n64 = tc.p64[i + 1] - tc.p64[i];
+                const int64_t n32 = tc.p32[i + 1] - tc.p32[i];
+                const int64_t cnt = n64 + n32;
+                sq += cnt;
+                s64 += n64;
+                pq += (int64_t)PX.ppn[i] * kp[cnt];
+            }
+            const int* l = qbx_class_l[cls];
+            const int64_t ncomp = (int64_t)qbx_ncart(l[0]) * qbx_ncart(l[1]) * qbx_ncart(l[2]) * qbx_ncart(l[3]);
+            if (shq) shq[cls] = sq;
+            if (cartq) cartq[cls] = sq * ncomp;
+            if (primq) primq[cls] = pq;
+            if (fp64q) fp64q[cls] = s64;
+        }
+    });
+}
+
+int qbx_set_timing(qbx_ctx* ctx, int32_t enable)
+{
+    return guarded([&] {
+        if (!ctx) throw QbxError(QBX_EINVAL, "null context");
+        ctx->timing = enable != 0;
+    });
+}
+
+int qbx_last_timing(qbx_ctx* ctx, float* ms, float* total)
+{
+    return guarded([&] {
+        if (!ctx) throw QbxError(QBX_EINVAL, "null context");
+        if (ctx->last_ms.empty()) throw QbxError(QBX_EINVAL, "no timed qbx_jk call yet");
+        if (ms)
+            for (int c = 0; c < QBX_N_CLASSES; ++c) ms[c] = ctx->last_ms[c];
+        if (total) *total = ctx->last_total;
+    });
+}
+
+int qbx_set_mixed_cut(qbx_ctx* ctx, double cut)
+{
+    return guarded([&] {
+        if (!ctx) throw QbxError(QBX_EINVAL, "null context");
+        if (!(cut >= 0.0)) throw QbxError(QBX_EINVAL, "cut must be >= 0");
+        ctx->mixed_cut = cut;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+// qbx_device.h -- device-side building blocks of libqbx_hip.so (gfx950).
+//
+// Boys function F_m(T): tabulated on T_i = i/10 (0 <= T < QBX_BOYS_TMAX) for
+// m = 0..QBX_BOYS_NCOL-1 together with exp(-T_i).  For T below the cut, the
+// highest needed order comes from an 8-term Taylor expansion about the nearest
+// grid point (|T - T_i| <= 0.05, truncation < 1e-15 relative) and the lower
+// orders from the stable downward recurrence
+//   F_{m-1}(T) = (2T F_m(T) + e^{-T}) / (2m - 1);
+// above the cut, F_0 = sqrt(pi/T)/2 and the upward recurrence without the
+// exponentially small e^{-T} term.  Everything here is FP64, also in the
+// mixed-precision kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define QBX_BOYS_DT 0.1
+#define QBX_BOYS_INVDT 10.0
+#define QBX_BOYS_TMAX 60.0
+#define QBX_BOYS_ROWS 602
+#define QBX_BOYS_NCOL 17     // F_0 .. F_16: highest order 8 plus 8 Taylor terms
+#define QBX_BOYS_TAYLOR 8
+
+#define QBX_MAXL 2           // largest shell angular momentum in the kernels
+#define QBX_MAXC 6           // Cartesian components of the largest shell
+
+struct DevShell {
+    double x, y, z;
+    int L;
+    int nprim;
+    int bf[QBX_MAXC];        // basis-function index of each component, -1 absent
+    double sc[QBX_MAXC];     // component scale (weight ratio to the shell's radial coefficients)
+};
+
+// Primitive-pair record of a shell pair (A,B): p = a+b, P the product centre,
+// PA = P - A, Kp = c_a c_b exp(-ab/p |A-B|^2) / p.
+struct PrimPair {
+    double p, inv2p, Px, Py, Pz, PAx, PAy, PAz, Kp, pad;
+};
+
+__host__ __device__ constexpr int qbx_ncart(int L) { return (L + 1) * (L + 2) / 2; }
+
+template <int L>
+__device__ __forceinline__ void qbx_boys(double T, double* __restrict__ F,
+                                         const double* __restrict__ tab,
+                                         const double* __restrict__ etab)
+{
+    if (T < QBX_BOYS_TMAX) {
+        const int i = __double2int_rn(T * QBX_BOYS_INVDT);
+        const double md = (double)i * QBX_BOYS_DT - T;   // -(T - T_i)
+        const double* __restrict__ row = tab + i * QBX_BOYS_NCOL + L;
+        double acc = row[QBX_BOYS_TAYLOR - 1];
+#pragma unroll
+        for (int k = QBX_BOYS_TAYLOR - 1; k >= 1; --k) acc = row[k - 1] + (md * (1.0 / k)) * acc;
+        F[L] = acc;
+        if (L > 0) {
+            double ex = 1.0;
+#pragma unroll
+            for (int k = 8; k >= 1; --k) ex = 1.0 + (md * (1.0 / k)) * ex;
+            const double e = etab[i] * ex;
+            const double t2 = 2.0 * T;
+#pragma unroll
+            for (int m = L - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
+        }
+    } else {
+        const double rt = rsqrt(T);
+        F[0] = 0.88622692545275801365 * rt;
+        const double h = 0.5 * rt * rt;
+#pragma unroll
+        for (int m = 0; m < L; ++m) F[m + 1] = F[m] * ((2 * m + 1) * h);
+    }
+}
+
+// Runtime-order Boys function (1e kernels): F_0..F_n, n <= 2*QBX_MAXL.
+__device__ __forceinline__ void qbx_boys_rt(int n, double T, double* __restrict__ F,
+                                            const double* __restrict__ tab,
+                                            const double* __restrict__ etab)
+{
+    if (T < QBX_BOYS_TMAX) {
+        const int i = __double2int_rn(T * QBX_BOYS_INVDT);
+        const double md = (double)i * QBX_BOYS_DT - T;
+        const double* __restrict__ row = tab + i * QBX_BOYS_NCOL + n;
+        double acc = row[QBX_BOYS_TAYLOR - 1];
+        for (int k = QBX_BOYS_TAYLOR - 1; k >= 1; --k) acc = row[k - 1] + (md * (1.0 / k)) * acc;
+        F[n] = acc;
+        double ex = 1.0;
+        for (int k = 8; k >= 1; --k) ex = 1.0 + (md * (1.0 / k)) * ex;
+        const double e = etab[i] * ex;
+        for (int m = n - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) / (2 * m + 1);
+    } else {
+        const double rt = rsqrt(T);
+        F[0] = 0.88622692545275801365 * rt;
+        const double h = 0.5 * rt * rt;
+        for (int m = 0; m < n; ++m) F[m + 1] = F[m] * ((2 * m + 1) * h);
+    }
+}
+
+// Cartesian component (i,j,k) of index c in shell L (order of Lexicons.jl:17-36).
+__host__ __device__ inline void qbx_cart(int L, int c, int* ijk)
+{
+    int n = 0;
+    for (int i = L; i >= 0; --i)
+        for (int j = L - i; j >= 0; --j) {
+            if (n == c) { ijk[0] = i; ijk[1] = j; ijk[2] = L - i - j; return; }
+            ++n;
+        }
+    ijk[0] = ijk[1] = ijk[2] = 0;
+}
+
+// 1D overlap recurrence (Obara-Saika): table S[i][j] of
+// int (x-A)^i (x-B)^j exp(-p (x-P)^2) dx / sqrt(pi/p),  i <= ni, j <= nj.
+#define QBX_OV_MAX 9
+__device__ inline void qbx_ov1d(double PA, double PB, double inv2p, int ni, int nj,
+                                double S[QBX_OV_MAX][QBX_OV_MAX])
+{
+    S[0][0] = 1.0;
+    for (int i = 0; i <= ni; ++i) {
+        if (i > 0) {
+            double v = PA * S[i - 1][0];
+            if (i > 1) v += inv2p * (i - 1) * S[i - 2][0];
+            S[i][0] = v;
+        }
+        for (int j = 1; j <= nj; ++j) {
+            double v = PB * S[i][j - 1];
+            if (i > 0) v += inv2p * i * S[i - 1][j - 1];
+            if (j > 1) v += inv2p * (j - 1) * S[i][j - 2];
+            S[i][j] = v;
+        }
+    }
+}

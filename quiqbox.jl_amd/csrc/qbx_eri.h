@@ -1,0 +1,275 @@
+// qbx_eri.h -- two-electron kernels of libqbx_hip.so (gfx950).
+//
+// One thread evaluates one contracted shell quartet of class (LA LB|LC LD):
+// the primitive loop runs the generated vertical recurrence (EriClass::vrr)
+// and sums [e0|f0] over primitive quartets; the generated horizontal
+// recurrence (EriClass::hrr) then yields (ab|cd).  The epilogue depends on
+// MODE: Fock digest (J/K with FP64 atomics), dense tensor, or Schwarz bound.
+// Each class is instantiated in its own generated translation unit
+// (gen/eri_<class>.hip) so that the classes compile in parallel.
+#pragma once
+#include "qbx.h"
+#include "qbx_common.h"
+#include "qbx_device.h"
+#include "qbx_eri_api.h"
+
+#include <cstdint>
+
+template <int LA, int LB, int LC, int LD, typename R> struct EriClass;
+
+namespace qbx {
+template <int LA, int LB, int LC, int LD, typename R, int MODE>
+__global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
+{
+    using Cls = EriClass<LA, LB, LC, LD, R>;
+    constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB), NC = qbx_ncart(LC), ND = qbx_ncart(LD);
+    constexpr int LT = LA + LB + LC + LD;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.total) return;
+    int bi, ki;
+    if (MODE == MODE_SCHWARZ) {
+        bi = (int)t;
+        ki = (int)t;
+    } else {
+        int lo = 0, hi = a.nbra;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (a.prefix[mid] <= t) lo = mid; else hi = mid;
+        }
+        bi = lo;
+        ki = a.koff[lo] + (int)(t - a.prefix[lo]);
+    }
+
+    R acc[Cls::NACC];
+#pragma unroll
+    for (int n = 0; n < Cls::NACC; ++n) acc[n] = (R)0;
+
+    const int bo = a.bra.ppo[bi], bn = a.bra.ppn[bi];
+    const int ko = a.ket.ppo[ki], kn = a.ket.ppn[ki];
+    for (int u = 0; u < bn; ++u) {
+        const PrimPair P = a.pp[bo + u];
+        for (int v = 0; v < kn; ++v) {
+            const PrimPair Q = a.pp[ko + v];
+            const double pq = P.p + Q.p;
+            const double s = rsqrt(pq);
+            const double inv = s * s;
+            const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
+            const double rho = P.p * Q.p * inv;
+            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+            // 2 pi^{5/2} (c c K / p)_{ab} (c c K / q)_{cd} / sqrt(p+q)
+            const double pref = 34.986836655249725693 * P.Kp * Q.Kp * s;
+            double Fd[LT + 1];
+            qbx_boys<LT>(T, Fd, a.boys, a.bexp);
+            R F[LT + 1];
+#pragma unroll
+            for (int m = 0; m <= LT; ++m) F[m] = (R)(Fd[m] * pref);
+            const double rho_p = Q.p * inv, rho_q = P.p * inv;
+            Cls::vrr(F, acc, (R)P.PAx, (R)P.PAy, (R)P.PAz,
+                     (R)(-rho_p * PQx), (R)(-rho_p * PQy), (R)(-rho_p * PQz),
+                     (R)Q.PAx, (R)Q.PAy, (R)Q.PAz,
+                     (R)(rho_q * PQx), (R)(rho_q * PQy), (R)(rho_q * PQz),
+                     (R)P.inv2p, (R)Q.inv2p, (R)(0.5 * inv), (R)rho_p, (R)rho_q);
+        }
+    }
+
+    const double* ABp = a.bra.AB + 3 * bi;
+    const double* CDp = a.ket.AB + 3 * ki;
+    R outR[NA * NB * NC * ND];
+    Cls::hrr(acc, outR, (R)ABp[0], (R)ABp[1], (R)ABp[2], (R)CDp[0], (R)CDp[1], (R)CDp[2]);
+
+    const int sA = a.bra.shA[bi], sB = a.bra.shB[bi];
+    const int sC = a.ket.shA[ki], sD = a.ket.shB[ki];
+    const DevShell& SA = a.sh[sA];
+    const DevShell& SB = a.sh[sB];
+    const DevShell& SC = a.sh[sC];
+    const DevShell& SD = a.sh[sD];
+
+    double I[NA * NB * NC * ND];
+#pragma unroll
+    for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                for (int id = 0; id < ND; ++id) {
+                    const int n = ((ia * NB + ib) * NC + ic) * ND + id;
+                    I[n] = (double)outR[n] * (SA.sc[ia] * SB.sc[ib]) * (SC.sc[ic] * SD.sc[id]);
+                }
+
+    if (MODE == MODE_SCHWARZ) {
+        double mx = 0.0;
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib) {
+                const double v = fabs(I[((ia * NB + ib) * NC + ia) * ND + ib]);
+                mx = v > mx ? v : mx;
+            }
+        a.qout[bi] = sqrt(mx);
+        return;
+    }
+
+    int fa[NA], fb[NB], fc[NC], fd[ND];
+#pragma unroll
+    for (int n = 0; n < NA; ++n) fa[n] = SA.bf[n];
+#pragma unroll
+    for (int n = 0; n < NB; ++n) fb[n] = SB.bf[n];
+#pragma unroll
+    for (int n = 0; n < NC; ++n) fc[n] = SC.bf[n];
+#pragma unroll
+    for (int n = 0; n < ND; ++n) fd[n] = SD.bf[n];
+
+    const size_t N = (size_t)a.N;
+    if (MODE == MODE_DENSE) {
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) {
+                        const size_t i = fa[ia], j = fb[ib], k = fc[ic], l = fd[id];
+                        if (fa[ia] < 0 || fb[ib] < 0 || fc[ic] < 0 || fd[id] < 0) continue;
+                        const double v = I[((ia * NB + ib) * NC + ic) * ND + id];
+                        a.eri[((i * N + j) * N + k) * N + l] = v;
+                        a.eri[((j * N + i) * N + k) * N + l] = v;
+                        a.eri[((i * N + j) * N + l) * N + k] = v;
+                        a.eri[((j * N + i) * N + l) * N + k] = v;
+                        a.eri[((k * N + l) * N + i) * N + j] = v;
+                        a.eri[((l * N + k) * N + i) * N + j] = v;
+                        a.eri[((k * N + l) * N + j) * N + i] = v;
+                        a.eri[((l * N + k) * N + j) * N + i] = v;
+                    }
+        return;
+    }
+
+    // ---- MODE_JK: digest with the degeneracy of the unique shell quartet ----
+    double f = 1.0;
+    if (sA == sB) f *= 0.5;
+    if (sC == sD) f *= 0.5;
+    if (a.same && bi == ki) f *= 0.5;
+    const double fj = 4.0 * f, fk = 2.0 * f;
+    // missing components carry weight 0 and point at function 0
+#pragma unroll
+    for (int n = 0; n < NA; ++n) fa[n] = fa[n] < 0 ? 0 : fa[n];
+#pragma unroll
+    for (int n = 0; n < NB; ++n) fb[n] = fb[n] < 0 ? 0 : fb[n];
+#pragma unroll
+    for (int n = 0; n < NC; ++n) fc[n] = fc[n] < 0 ? 0 : fc[n];
+#pragma unroll
+    for (int n = 0; n < ND; ++n) fd[n] = fd[n] < 0 ? 0 : fd[n];
+
+    for (int s = 0; s < a.nset; ++s) {
+        const double* __restrict__ DJ = a.D + (size_t)s * N * N;
+        const double* __restrict__ DK = a.D + (size_t)(a.nset + s) * N * N;
+        double* Js = a.J + (size_t)s * N * N;
+        double* Ks = a.K + (size_t)s * N * N;
+        double dab[NA * NB], dcd[NC * ND];
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib) dab[ia * NB + ib] = DJ[fa[ia] * N + fb[ib]];
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = DJ[fc[ic] * N + fd[id]];
+        // J_ab += 4f sum_cd I D_cd ;  J_cd += 4f sum_ab I D_ab
+        double jcd[NC * ND];
+#pragma unroll
+        for (int n = 0; n < NC * ND; ++n) jcd[n] = 0.0;
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib) {
+                double jab = 0.0;
+                const double d = dab[ia * NB + ib];
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) {
+                        const double v = I[((ia * NB + ib) * NC + ic) * ND + id];
+                        jab += v * dcd[ic * ND + id];
+                        jcd[ic * ND + id] += v * d;
+                    }
+                atomicAdd(Js + fa[ia] * N + fb[ib], fj * jab);
+            }
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) atomicAdd(Js + fc[ic] * N + fd[id], fj * jcd[ic * ND + id]);
+
+        // K_ad += 2f sum_bc I D_bc ; K_bd += 2f sum_ac I D_ac ;
+        // K_ac += 2f sum_bd I D_bd ; K_bc += 2f sum_ad I D_ad
+        double dbc[NB * NC], dac[NA * NC], dbd[NB * ND], dad[NA * ND];
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) dbc[ib * NC + ic] = DK[fb[ib] * N + fc[ic]];
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) dac[ia * NC + ic] = DK[fa[ia] * N + fc[ic]];
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) dbd[ib * ND + id] = DK[fb[ib] * N + fd[id]];
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) dad[ia * ND + id] = DK[fa[ia] * N + fd[id]];
+        double kad[NA * ND], kbd[NB * ND], kac[NA * NC], kbc[NB * NC];
+#pragma unroll
+        for (int n = 0; n < NA * ND; ++n) kad[n] = 0.0;
+#pragma unroll
+        for (int n = 0; n < NB * ND; ++n) kbd[n] = 0.0;
+#pragma unroll
+        for (int n = 0; n < NA * NC; ++n) kac[n] = 0.0;
+#pragma unroll
+        for (int n = 0; n < NB * NC; ++n) kbc[n] = 0.0;
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) {
+                        const double v = I[((ia * NB + ib) * NC + ic) * ND + id];
+                        kad[ia * ND + id] += v * dbc[ib * NC + ic];
+                        kbd[ib * ND + id] += v * dac[ia * NC + ic];
+                        kac[ia * NC + ic] += v * dbd[ib * ND + id];
+                        kbc[ib * NC + ic] += v * dad[ia * ND + id];
+                    }
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) atomicAdd(Ks + fa[ia] * N + fd[id], fk * kad[ia * ND + id]);
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) atomicAdd(Ks + fb[ib] * N + fd[id], fk * kbd[ib * ND + id]);
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) atomicAdd(Ks + fa[ia] * N + fc[ic], fk * kac[ia * NC + ic]);
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) atomicAdd(Ks + fb[ib] * N + fc[ic], fk * kbc[ib * NC + ic]);
+    }
+}
+
+
+template <int LA, int LB, int LC, int LD, typename R, int MODE>
+void launch_eri(const EriArgs& a, hipStream_t s)
+{
+    if (a.total <= 0) return;
+    const int threads = 256;
+    const int64_t blocks = (a.total + threads - 1) / threads;
+    hipLaunchKernelGGL((eri_kernel<LA, LB, LC, LD, R, MODE>), dim3((unsigned)blocks), dim3(threads), 0, s, a);
+    QBX_HIP(hipGetLastError());
+}
+
+}  // namespace qbx

@@ -1,0 +1,45 @@
+// qbx_eri_api.h -- argument block and launcher declaration of the ERI kernels.
+#pragma once
+#include "qbx_device.h"
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace qbx {
+enum EriMode { MODE_JK = 0, MODE_DENSE = 1, MODE_SCHWARZ = 2 };
+
+struct PairClassDev {
+    int n;
+    const int* shA;
+    const int* shB;
+    const int* ppo;
+    const int* ppn;
+    const double* AB;
+};
+
+struct EriArgs {
+    PairClassDev bra, ket;
+    const PrimPair* pp;
+    const DevShell* sh;
+    const double* boys;
+    const double* bexp;
+    const int64_t* prefix;
+    const int* koff;
+    int64_t total;
+    int nbra;
+    int same;
+    const double* D;   // [2*nset][N][N]: DJ sets then DK sets
+    double* J;         // [nset][N][N]
+    double* K;
+    int nset;
+    int N;
+    double* eri;       // dense mode, N^4
+    double* qout;      // Schwarz mode, per bra pair
+};
+
+
+// Defined (explicitly instantiated) in the generated per-class translation units.
+template <int LA, int LB, int LC, int LD, typename R, int MODE>
+void launch_eri(const EriArgs& a, hipStream_t s);
+
+typedef void (*EriLaunch)(const EriArgs&, hipStream_t);
+}  // namespace qbx

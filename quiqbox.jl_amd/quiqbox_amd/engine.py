@@ -1,0 +1,183 @@
+"""Device context and basis handles over the C-ABI.
+
+`Context` owns one GPU and one stream (qbx_init); `GPUBasis` owns a
+qbx_basis (qbx_basis_create) and exposes the integral and Fock-build entry
+points.  Bases whose functions mix centres or Cartesian exponents inside one
+contraction are evaluated in an auxiliary one-primitive-per-function basis and
+transformed with phi = chi T.
+"""
+import ctypes
+import threading
+
+import numpy as np
+
+from . import _lib
+from .basis import FlatBasis, expand_to_standard, flatten_basis
+
+_default_ctx = None
+_ctx_lock = threading.Lock()
+
+
+class Context:
+    def __init__(self, device=0):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(L.qbx_init(int(device), ctypes.byref(h)))
+        self.h = h
+        self.device = device
+        self.lock = threading.Lock()
+
+    def close(self):
+        if self.h:
+            _lib.load().qbx_finalize(self.h)
+            self.h = None
+
+    def set_timing(self, enable=True):
+        _lib.check(_lib.load().qbx_set_timing(self.h, int(bool(enable))))
+
+    def last_timing(self):
+        n = _lib.load().qbx_n_classes()
+        ms = (ctypes.c_float * n)()
+        tot = ctypes.c_float()
+        _lib.check(_lib.load().qbx_last_timing(self.h, ms, ctypes.byref(tot)))
+        return np.array(ms[:], dtype=np.float64), float(tot.value)
+
+    def set_mixed_cut(self, cut):
+        _lib.check(_lib.load().qbx_set_mixed_cut(self.h, float(cut)))
+
+
+def default_context():
+    global _default_ctx
+    with _ctx_lock:
+        if _default_ctx is None:
+            _default_ctx = Context(0)
+        return _default_ctx
+
+
+def class_table():
+    L = _lib.load()
+    n = L.qbx_n_classes()
+    t = np.zeros(4 * n, dtype=np.int32)
+    _lib.check(L.qbx_class_table(_lib.ip(t)))
+    return t.reshape(n, 4)
+
+
+class GPUBasis:
+    """A basis set resident on the GPU."""
+
+    def __init__(self, orbs_or_flat, ctx=None):
+        self.ctx = ctx or default_context()
+        flat = orbs_or_flat if isinstance(orbs_or_flat, FlatBasis) else flatten_basis(orbs_or_flat)
+        self.flat = flat
+        self.n = flat.n_bf
+        if flat.is_standard():
+            self.T = None
+            dev = flat
+        else:
+            dev, self.T = expand_to_standard(flat)
+        self.n_dev = dev.n_bf
+        L = _lib.load()
+        self._keep = [_lib.f64(dev.prim_cen).reshape(-1), _lib.f64(dev.prim_xpn), _lib.i32(dev.prim_ang).reshape(-1),
+                      _lib.i32(dev.bf_ptr), _lib.i32(dev.bf_prim), _lib.f64(dev.bf_w)]
+        c, x, a, p, q, w = self._keep
+        h = ctypes.c_void_p()
+        _lib.check(L.qbx_basis_create(self.ctx.h, dev.n_prim, _lib.dp(c), _lib.dp(x), _lib.ip(a), dev.n_bf,
+                                      _lib.ip(p), _lib.ip(q), _lib.dp(w), ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib.load().qbx_basis_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def info(self):
+        v = [ctypes.c_int32() for _ in range(4)]
+        _lib.check(_lib.load().qbx_basis_info(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(n_bf=v[0].value, n_shell=v[1].value, n_pair=v[2].value, max_l=v[3].value)
+
+    # ---- helpers for the auxiliary-basis transform ----
+    def _mat_out(self, fn, *args):
+        M = np.zeros((self.n_dev, self.n_dev))
+        with self.ctx.lock:
+            _lib.check(fn(self.ctx.h, self.h, *args, _lib.dp(M)))
+        if self.T is not None:
+            M = self.T.T @ M @ self.T
+        return M
+
+    def overlap(self):
+        return self._mat_out(_lib.load().qbx_overlap)
+
+    def kinetic(self):
+        return self._mat_out(_lib.load().qbx_kinetic)
+
+    def nuclear(self, Z, R):
+        Z = _lib.f64(Z)
+        R = _lib.f64(R).reshape(-1)
+        return self._mat_out(_lib.load().qbx_nuclear, len(Z), _lib.dp(Z), _lib.dp(R))
+
+    def core_hamiltonian(self, Z, R):
+        Z = _lib.f64(Z)
+        R = _lib.f64(R).reshape(-1)
+        return self._mat_out(_lib.load().qbx_core_hamiltonian, len(Z), _lib.dp(Z), _lib.dp(R))
+
+    def multipole(self, center, degree):
+        c = _lib.f64(center)
+        d = _lib.i32(degree)
+        return self._mat_out(_lib.load().qbx_multipole, _lib.dp(c), _lib.ip(d))
+
+    def eri_dense(self):
+        n = self.n_dev
+        E = np.zeros((n, n, n, n))
+        with self.ctx.lock:
+            _lib.check(_lib.load().qbx_eri_dense(self.ctx.h, self.h, _lib.dp(E)))
+        if self.T is not None:
+            T = self.T
+            E = np.einsum("abcd,ai,bj,ck,dl->ijkl", E, T, T, T, T, optimize=True)
+        return E
+
+    def jk(self, DJ, DK, thr=1e-12, precision=_lib.QBX_PREC_FP64):
+        """G[s] = J[DJ[s]] - K[DK[s]] for stacked (n_set, n, n) densities."""
+        DJ = np.asarray(DJ, dtype=np.float64)
+        DK = np.asarray(DK, dtype=np.float64)
+        single = DJ.ndim == 2
+        if single:
+            DJ = DJ[None]
+            DK = DK[None]
+        if DJ.shape != DK.shape or DJ.shape[1:] != (self.n, self.n):
+            raise ValueError("density shape %s / %s does not match the basis size %d" % (DJ.shape, DK.shape, self.n))
+        if self.T is not None:
+            T = self.T
+            DJ = np.einsum("ai,sij,bj->sab", T, DJ, T)
+            DK = np.einsum("ai,sij,bj->sab", T, DK, T)
+        DJ = np.ascontiguousarray(DJ)
+        DK = np.ascontiguousarray(DK)
+        G = np.zeros_like(DJ)
+        with self.ctx.lock:
+            _lib.check(_lib.load().qbx_jk(self.ctx.h, self.h, DJ.shape[0], _lib.dp(DJ), _lib.dp(DK), _lib.dp(G),
+                                          float(thr), int(precision)))
+        if self.T is not None:
+            G = np.einsum("ai,sab,bj->sij", self.T, G, self.T)
+        return G[0] if single else G
+
+    def two_body(self, threshold=1e-12, precision=_lib.QBX_PREC_FP64):
+        """The Fock-build operator of this basis (direct, Schwarz-screened)."""
+        from .hartree_fock import DirectJK
+        return DirectJK(self, threshold, precision)
+
+    def jk_device(self, dDJ_ptr, dDK_ptr, dG_ptr, n_set, thr=1e-12, precision=_lib.QBX_PREC_FP64, stream=None):
+        if self.T is not None:
+            raise ValueError("device-pointer Fock builds need a standard (single-centre) basis")
+        with self.ctx.lock:
+            _lib.check(_lib.load().qbx_jk_device(self.ctx.h, self.h, int(n_set), ctypes.c_void_p(dDJ_ptr),
+                                                 ctypes.c_void_p(dDK_ptr), ctypes.c_void_p(dG_ptr), float(thr),
+                                                 int(precision), ctypes.c_void_p(stream or 0)))
+
+    def jk_stats(self, thr=1e-12, cut=0.0):
+        n = _lib.load().qbx_n_classes()
+        arrs = [np.zeros(n, dtype=np.int64) for _ in range(4)]
+        _lib.check(_lib.load().qbx_jk_stats(self.ctx.h, self.h, float(thr), float(cut),
+                                            *[a.ctypes.data_as(_lib._pl) for a in arrs]))
+        return dict(shell_quartets=arrs[0], cart_quartets=arrs[1], prim_quartets=arrs[2], fp64_shell_quartets=arrs[3])

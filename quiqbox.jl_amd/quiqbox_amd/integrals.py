@@ -1,0 +1,117 @@
+"""Public integral functions with Quiqbox's names and argument meaning
+(src/Integration/Interface.jl).  Every value is computed by libqbx_hip.so.
+
+The `lazyCompute` keyword is accepted for drop-in compatibility; the GPU path
+evaluates every requested integral directly (Quiqbox's lazy and eager paths
+are bit-identical by contract, Coulomb-test.jl:102,149).
+"""
+import numpy as np
+
+from .engine import GPUBasis
+from .particles import NuclearCluster, getCharge
+
+
+def _basis(orbs):
+    return GPUBasis(list(orbs))
+
+
+def _nuclei(nucs, coords):
+    if isinstance(nucs, NuclearCluster):
+        return nucs.charges, nucs.coord_array
+    Z = np.array([getCharge(s) for s in nucs], dtype=np.float64)
+    R = np.array([tuple(c) for c in coords], dtype=np.float64).reshape(-1, 3)
+    if len(Z) != len(R):
+        raise ValueError("nuclei and coordinates differ in length")
+    return Z, R
+
+
+def overlaps(basisSet, lazyCompute=True):
+    """Interface.jl:46-54"""
+    return _basis(basisSet).overlap()
+
+
+def overlap(orbL, orbR, lazyCompute=True):
+    """Interface.jl:21-33"""
+    return _basis([orbL, orbR]).overlap()[0, 1]
+
+
+def multipoleMoments(center, degrees, basisSet, lazyCompute=True):
+    """Interface.jl:94-105"""
+    return _basis(basisSet).multipole(center, degrees)
+
+
+def multipoleMoment(center, degrees, orbL, orbR, lazyCompute=True):
+    """Interface.jl:71-80"""
+    return _basis([orbL, orbR]).multipole(center, degrees)[0, 1]
+
+
+def elecKinetics(basisSet, lazyCompute=True):
+    """Interface.jl:143-153"""
+    return _basis(basisSet).kinetic()
+
+
+def elecKinetic(orbL, orbR, lazyCompute=True):
+    """Interface.jl:120-130"""
+    return _basis([orbL, orbR]).kinetic()[0, 1]
+
+
+def nucAttractions(nucs, coords_or_basis, basisSet=None, lazyCompute=True):
+    """nucAttractions(nucs, coords, bs) or nucAttractions(nucInfo, bs) (Interface.jl:236-256)"""
+    if basisSet is None:
+        Z, R = _nuclei(nucs, None)
+        basisSet = coords_or_basis
+    else:
+        Z, R = _nuclei(nucs, coords_or_basis)
+    return _basis(basisSet).nuclear(Z, R)
+
+
+def nucAttraction(nucs, *args, lazyCompute=True):
+    """nucAttraction(nucs, coords, orbL, orbR) or nucAttraction(nucInfo, orbL, orbR) (Interface.jl:186-205)"""
+    if isinstance(nucs, NuclearCluster):
+        orbL, orbR = args
+        Z, R = _nuclei(nucs, None)
+    else:
+        coords, orbL, orbR = args
+        Z, R = _nuclei(nucs, coords)
+    return _basis([orbL, orbR]).nuclear(Z, R)[0, 1]
+
+
+def coreHamiltonian(nucs, coords_or_basis, basisSet=None, lazyCompute=True):
+    """kinetic + nuclear attraction (Interface.jl:288-312, Samplers.jl:110-131)"""
+    if basisSet is None:
+        Z, R = _nuclei(nucs, None)
+        basisSet = coords_or_basis
+    else:
+        Z, R = _nuclei(nucs, coords_or_basis)
+    return _basis(basisSet).core_hamiltonian(Z, R)
+
+
+def elecRepulsions(basisSet, lazyCompute=True):
+    """Dense (ij|kl) tensor, chemists' notation (Interface.jl:345-365)."""
+    return _basis(basisSet).eri_dense()
+
+
+def elecRepulsion(orb1, orb2, orb3, orb4, lazyCompute=True):
+    """(12|34) (Interface.jl:320-342)"""
+    return _basis([orb1, orb2, orb3, orb4]).eri_dense()[0, 1, 2, 3]
+
+
+def changeOrbitalBasis(DbodyInt, *C):
+    """Transform one- or two-body AO integrals with orbital coefficients
+    (Interface.jl:376-406).  With two coefficient matrices and a 4-index
+    tensor: (ERI_alpha, ERI_beta, J_alphabeta)."""
+    A = np.asarray(DbodyInt)
+    if len(C) == 1:
+        c = np.asarray(C[0])
+        if A.ndim == 2:
+            return c.T @ A @ c
+        if A.ndim == 4:
+            return np.einsum("abcd,ai,bj,ck,dl->ijkl", A, c, c, c, c, optimize=True)
+        raise ValueError("one- or two-body integrals expected")
+    if len(C) == 2 and A.ndim == 4:
+        c1, c2 = (np.asarray(x) for x in C)
+        e1 = changeOrbitalBasis(A, c1)
+        e2 = changeOrbitalBasis(A, c2)
+        jab = np.einsum("abcd,am,bm,cn,dn->mn", A, c1, c1, c2, c2, optimize=True)
+        return e1, e2, jab
+    raise ValueError("unsupported changeOrbitalBasis arguments")

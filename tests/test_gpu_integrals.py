@@ -1,0 +1,114 @@
+"""GPU one- and two-electron integrals (libqbx_hip.so) against the oracle and
+the reference's literals.  ERIs: 1e-12 absolute."""
+import numpy as np
+import pytest
+
+import oracle
+import quiqbox_amd as qb
+from helpers import OracleEngine, contracted, lih_basis, literals, molecule_basis, random_spd_basis
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print("[gpu-test] " + msg, flush=True)
+
+
+def _compare_all(bs, Z, R, tol_eri=1e-12, tol_1e=1e-12, dense=True, label=""):
+    _say("integrals %s n_bf=%d" % (label, len(bs)))
+    gb = qb.GPUBasis(bs)
+    eng = OracleEngine(gb.flat)
+    np.testing.assert_allclose(gb.overlap(), eng.overlap(), rtol=0, atol=tol_1e)
+    np.testing.assert_allclose(gb.kinetic(), eng.kinetic(), rtol=0, atol=tol_1e)
+    np.testing.assert_allclose(gb.nuclear(Z, R), eng.nuclear(Z, R), rtol=0, atol=10 * tol_1e)
+    H = gb.core_hamiltonian(Z, R)
+    np.testing.assert_allclose(H, eng.core_hamiltonian(Z, R), rtol=0, atol=10 * tol_1e)
+    if dense:
+        E = gb.eri_dense()
+        Eo = eng.eri()
+        err = np.max(np.abs(E - Eo))
+        _say("  max |ERI - ERI_oracle| = %.3e" % err)
+        assert err <= tol_eri
+    return gb, eng
+
+
+def test_lih_against_reference_literals():
+    L = literals()["lih"]
+    Z = [qb.getCharge(s) for s in L["nucs"]]
+    gb, eng = _compare_all(lih_basis(), Z, L["coords"], label="LiH")
+    np.testing.assert_allclose(gb.overlap().ravel(), L["S"], rtol=1.5e-8, atol=1e-12)
+    np.testing.assert_allclose(gb.nuclear(Z, L["coords"]).ravel(), L["V"], rtol=1.5e-8, atol=1e-12)
+    E = gb.eri_dense()
+    for perm in ["jikl", "ijlk", "klij", "lkji"]:
+        np.testing.assert_array_equal(E, np.einsum("ijkl->" + perm, E))
+
+
+def test_h2o_sto3g_and_h2o2_631g():
+    L = literals()["hho_sto3g"]
+    bs = molecule_basis(L["nucs"], L["coords"], "STO-3G")
+    _compare_all(bs, [qb.getCharge(s) for s in L["nucs"]], L["coords"], label="HHO STO-3G")
+    L2 = literals()["h2o2_631g"]
+    coords = [tuple(L2["angstrom_to_bohr"] * x for x in c) for c in L2["coords_angstrom"]]
+    bs2 = molecule_basis(L2["nucs"], coords, "6-31G")
+    _compare_all(bs2, [qb.getCharge(s) for s in L2["nucs"]], coords, label="H2O2 6-31G")
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_random_spd_clusters_all_classes(seed):
+    coords, bs = random_spd_basis(seed, n_atoms=3 if seed < 2 else 4)
+    Z = [1.0 + k for k in range(len(coords))]
+    _compare_all(bs, Z, coords, label="spd seed %d" % seed)
+
+
+def test_li_321g_overlap_literal():
+    L = literals()["li_321g_overlap"]
+    bs = qb.genGaussTypeOrbSeq(tuple(L["center"]), "Li", "3-21G")
+    _say("Li 3-21G overlaps")
+    np.testing.assert_allclose(qb.overlaps(bs).ravel(), L["S"], rtol=0, atol=1e-14)
+
+
+def test_interface_functions_and_kinetic_literal():
+    L = literals()["interface"]
+    cgf1, cgf2 = contracted(L["cgf1"]), contracted(L["cgf2"])
+    _say("interface functions")
+    np.testing.assert_allclose(qb.overlaps([cgf1, cgf2]), L["S"], rtol=1e-13)
+    assert qb.overlap(cgf1, cgf2) == pytest.approx(L["S"][0][1], rel=1e-13)
+    nuc = qb.NuclearCluster(L["nucs"], L["coords"])
+    H = qb.coreHamiltonian(L["nucs"], L["coords"], [cgf1, cgf2])
+    np.testing.assert_allclose(H, qb.nucAttractions(nuc, [cgf1, cgf2]) + qb.elecKinetics([cgf1, cgf2]),
+                               rtol=0, atol=1e-14)
+    K = literals()["kinetic"]["contracted"]
+    assert qb.elecKinetic(contracted(K), contracted(K)) == pytest.approx(K["value"], rel=1e-12)
+    (p, v) = literals()["kinetic"]["prim"]
+    pg = qb.genGaussTypeOrb(tuple(p[0]), p[1], tuple(p[2]))
+    assert qb.elecKinetic(pg, pg) == pytest.approx(v, rel=1e-12)
+
+
+def test_multipoles_against_oracle():
+    coords, bs = random_spd_basis(5, n_atoms=2)
+    gb = qb.GPUBasis(bs)
+    eng = OracleEngine(gb.flat)
+    for deg in [(0, 0, 0), (1, 0, 0), (0, 1, 1), (2, 0, 1), (1, 2, 3)]:
+        _say("multipole %s" % (deg,))
+        M = gb.multipole((0.3, -0.2, 0.5), deg)
+        Mo = eng.ob.multipole((0.3, -0.2, 0.5), deg)
+        np.testing.assert_allclose(M, Mo, rtol=1e-12, atol=1e-12)
+
+
+def test_primitive_literals_through_single_primitive_bases():
+    # single unnormalized primitives up to d: the reference's primitive ERI literal with s functions
+    g = literals()["prim_gtos"]
+    p3, p4 = (qb.genGaussTypeOrb(tuple(g[k][0]), g[k][1], tuple(g[k][2])) for k in ("3", "4"))
+    _say("primitive (33|44)")
+    assert qb.elecRepulsion(p3, p3, p4, p4) == pytest.approx(0.7291219052871128, rel=1e-13)
+    p2, p6 = (qb.genGaussTypeOrb(tuple(g[k][0]), g[k][1], tuple(g[k][2])) for k in ("2", "6"))
+    V = qb.nucAttractions(["He", "H"], [(0, 0, 0), (0, 0, 0)], [p2, p6, p3])
+    np.testing.assert_allclose(V[2, 1], -3 * 0.11995218441914622, rtol=1e-13)
+
+
+def test_nonstandard_composite_basis():
+    mixed = qb.CompositeOrb([qb.PrimitiveOrb((0, 0, 0), 1.0), qb.PrimitiveOrb((0, 0, 1.0), 1.3, (1, 0, 0))],
+                            [1.0, -0.4])
+    bs = [mixed] + qb.genGaussTypeOrbSeq((0.3, 0.0, 0.0), "H", "STO-3G") + \
+        qb.genGaussTypeOrbSeq((0.0, 1.2, 0.0), "Li", "3-21G")
+    _compare_all(bs, [1.0, 3.0], [(0.3, 0, 0), (0, 1.2, 0)], label="non-standard")

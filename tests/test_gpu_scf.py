@@ -1,0 +1,78 @@
+"""runHartreeFock on the GPU engine against the oracle-driven SCF (1e-10 Eh)
+and the reference's literals."""
+import numpy as np
+import pytest
+
+import quiqbox_amd as qb
+from helpers import OracleEngine, literals, molecule_basis
+from quiqbox_amd.systems import h10_sto3g, random_cluster
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print("[gpu-test] " + msg, flush=True)
+
+
+def test_config1_h10_rhf_matches_oracle():
+    nuc, bs = h10_sto3g()
+    cfg = qb.HFconfig(initial="CoreH", strategy=qb.SCFconfig(("DIIS",), (1e-10,)))
+    _say("config 1 H10 GPU SCF")
+    res = qb.runHartreeFock(nuc, bs, cfg, printInfo=False)
+    ref = qb.runHartreeFock(nuc, OracleEngine(bs), cfg, printInfo=False)
+    _say("  E_gpu %.12f E_oracle %.12f" % (res.energy[0], ref.energy[0]))
+    assert res.converged and ref.converged
+    assert abs(res.energy[0] - ref.energy[0]) < 1e-10
+
+
+def test_hho_rhf_uhf_against_reference_and_oracle():
+    L = literals()["hho_sto3g"]
+    nuc = qb.NuclearCluster(L["nucs"], L["coords"])
+    bs = molecule_basis(L["nucs"], L["coords"], "STO-3G")
+    scf = qb.SCFconfig(("ADIIS", "DIIS", "EDIIS", "DD"), (1e-4, 1e-8, 1e-10, 1e-15))
+    _say("HHO RHF")
+    res = qb.runHartreeFock(nuc, bs, qb.HFconfig(initial="CoreH", strategy=scf), printInfo=False)
+    assert res.energy[0] == pytest.approx(L["E_rhf"], abs=L["tol_E"])
+    np.testing.assert_allclose(res.fock[0], L["fock_rhf"], atol=L["tol_E"])
+    ref = qb.runHartreeFock(nuc, OracleEngine(bs), qb.HFconfig(initial="CoreH", strategy=scf), printInfo=False)
+    assert abs(res.energy[0] - ref.energy[0]) < 1e-10
+    _say("HHO UHF")
+    ures = qb.runHartreeFock(nuc, bs, qb.HFconfig(qb.UOHartreeFock(), initial="GWH", strategy=scf), printInfo=False)
+    assert ures.energy[0] == pytest.approx(L["E_uhf"], abs=100 * L["tol_E"])
+
+
+def test_h2o2_631g_default_config():
+    L = literals()["h2o2_631g"]
+    coords = [tuple(L["angstrom_to_bohr"] * x for x in c) for c in L["coords_angstrom"]]
+    nuc = qb.NuclearCluster(L["nucs"], coords)
+    bs = molecule_basis(nuc.symbols, nuc.coords, "6-31G")
+    cfg = qb.HFconfig(initial="SAD", strategy=qb.SCFconfig(threshold=L["threshold"], secondaryConvRatio=(5, 5)))
+    _say("H2O2 6-31G SAD default strategy")
+    res = qb.runHartreeFock(nuc, bs, cfg, printInfo=False)
+    assert res.converged
+    assert res.energy[0] == pytest.approx(L["E_elec"], abs=5 * L["threshold"])
+
+
+def test_h2_321g_curve_points():
+    L = literals()["h2_321g_curve"]
+    for k in (2, 6, 20, 70):
+        R = L["bond_lengths"][k]
+        coords = [(0.0, 0.0, 0.0), (R, 0.0, 0.0)]
+        nuc = qb.NuclearCluster(["H", "H"], coords)
+        bs = molecule_basis(["H", "H"], coords, "3-21G")
+        for HFT, key in ((qb.RCHartreeFock(), "rhf"), (qb.UOHartreeFock(), "uhf")):
+            _say("H2 R=%.1f %s" % (R, key))
+            res = qb.runHartreeFock(nuc, bs, qb.HFconfig(HFT, initial="CoreH", maxStep=300), printInfo=False)
+            assert sum(res.energy) == pytest.approx(L[key][k], abs=10 * 7.5e-8)
+
+
+def test_config2_small_cluster_matches_oracle():
+    """16-atom cluster of the config-2 generator (64 functions): GPU direct SCF
+    vs oracle dense-tensor SCF from the same core-Hamiltonian guess."""
+    nuc, bs = random_cluster(16, 3, box=10.0)
+    cfg = qb.HFconfig(initial="CoreH", strategy=qb.SCFconfig(("DIIS",), (1e-10,)), maxStep=100)
+    _say("16-atom cluster GPU SCF n_bf=%d" % len(bs))
+    res = qb.runHartreeFock(nuc, bs, cfg, printInfo=False, schwarzThreshold=0.0)
+    ref = qb.runHartreeFock(nuc, OracleEngine(bs), cfg, printInfo=False)
+    _say("  E_gpu %.12f E_oracle %.12f conv %s %s" % (res.energy[0], ref.energy[0], res.converged, ref.converged))
+    assert abs(res.energy[0] - ref.energy[0]) < 1e-9
