@@ -63,6 +63,7 @@ def lib():
         L.qor_eri_dense.argtypes = [bp, _pd]
         L.qor_eri_list.argtypes = [bp, _i64, _pi, _pd, _i32]
         L.qor_gcore.argtypes = [_i32, _pd, _pd, _pd, _pd]
+        L.qor_set_reference_orientation.argtypes = [_i32]
         _lib = L
     return _lib
 
@@ -82,6 +83,12 @@ def _f64(x, n=None):
 
 def _i32a(x):
     return np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+
+
+def set_reference_orientation(on):
+    """True: primitive quartets in the reference's canonical orientation
+    (bit-level restatement); False (default): numerically stable orientation."""
+    lib().qor_set_reference_orientation(1 if on else 0)
 
 
 def boys(x, n):

@@ -526,14 +526,32 @@ void qor_multipole_matrix(const qor_basis* b, const double* M, const int32_t* de
     matrix1(b, out, p1_multipole, &c);
 }
 
-/* primitive quartet with the index canonicalization of getIntegralIndexPair
-   (Framework.jl:335-357): i >= j, k >= l, (i,j) >= (k,l) */
+/* Orientation of a primitive quartet.  The reference canonicalizes the index
+   layout (getIntegralIndexPair, Framework.jl:335-357: i >= j, k >= l,
+   (i,j) >= (k,l)) before evaluating it.  The electron-transfer step of the
+   per-axis scheme scales terms by p/q (exponent sums of the two electrons), so
+   an orientation with p >> q loses digits (up to ~1e-9 absolute for mixed
+   tight/diffuse d quartets against a 40-digit McMurchie-Davidson evaluation).
+   By default the oracle evaluates the same integral in a stable orientation:
+   within each pair the primitive with the larger exponent first, and the pair
+   with the smaller exponent sum on electron 1 (p <= q); against 40-digit
+   McMurchie-Davidson values this keeps normalized (dd|dd) errors near 1e-16.
+   qor_set_reference_orientation(1) restores the reference's orientation. */
+static int g_reference_orientation = 0;
+
+void qor_set_reference_orientation(int32_t on) { g_reference_orientation = on != 0; }
+
 static double prim_eri_canon(const qor_basis* b, int32_t i, int32_t j, int32_t k, int32_t l)
 {
     int32_t t;
     if (i < j) { t = i; i = j; j = t; }
     if (k < l) { t = k; k = l; l = t; }
     if (i < k || (i == k && j < l)) { t = i; i = k; k = t; t = j; j = l; l = t; }
+    if (!g_reference_orientation) {
+        if (b->xpn[i] < b->xpn[j]) { t = i; i = j; j = t; }
+        if (b->xpn[k] < b->xpn[l]) { t = k; k = l; l = t; }
+        if (b->xpn[i] + b->xpn[j] > b->xpn[k] + b->xpn[l]) { t = i; i = k; k = t; t = j; j = l; l = t; }
+    }
     return qor_prim_eri(PRIM(b, i), PRIM(b, j), PRIM(b, k), PRIM(b, l));
 }
 

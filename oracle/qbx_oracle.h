@@ -64,6 +64,9 @@ void   qor_nuclear_matrix(const qor_basis* b, int32_t n_nuc, const double* Z,
                           const double* R, double* V);                             /* Samplers.jl:56 */
 void   qor_multipole_matrix(const qor_basis* b, const double* M, const int32_t* deg,
                             double* out);
+/* 1: evaluate primitive quartets in the reference's canonical orientation;
+   0 (default): larger exponent first in each pair, smaller exponent sum on electron 1 */
+void   qor_set_reference_orientation(int32_t on);
 double qor_eri_quartet(const qor_basis* b, int32_t i, int32_t j, int32_t k, int32_t l); /* :526 */
 void   qor_eri_dense(const qor_basis* b, double* eri);                             /* :640 */
 /* evaluate a list of contracted quartets (4*n indices); threads<=0 -> 1 */

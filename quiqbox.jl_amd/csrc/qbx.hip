@@ -365,6 +365,8 @@ int class_pc(int cls, int side)
     return pair_class_index(qbx_class_l[cls][2 * side], qbx_class_l[cls][2 * side + 1]);
 }
 
+bool X_is_Y(int cls) { return class_pc(cls, 0) == class_pc(cls, 1); }
+
 PairClassDev pcdev(const PairClass& c)
 {
     PairClassDev d;
@@ -1049,10 +1051,19 @@ int qbx_jk_stats(qbx_ctx* ctx, qbx_basis* b, double thr, double cut, int64_t* sh
             const PairClass& PX = b->pc[class_pc(cls, 0)];
             const PairClass& PY = b->pc[class_pc(cls, 1)];
             const int nx = PX.n();
-            // prefix sums of ket primitive-pair counts in sorted order
-            std::vector<int64_t> kp(PY.n() + 1, 0);
-            for (int j = 0; j < PY.n(); ++j) kp[j + 1] = kp[j] + PY.ppn[j];
-            int64_t sq = 0, pq = 0, s64 = 0;
+            // prefix sums over kets (sorted order) of primitive-pair counts and of
+            // unique basis-function pairs (n(n+1)/2 for a diagonal shell pair)
+            const int* l = qbx_class_l[cls];
+            auto upairs = [&](const PairClass& P, int k, int la, int lb) -> int64_t {
+                const int64_t na = qbx_ncart(la), nb = qbx_ncart(lb);
+                return P.shA[k] == P.shB[k] ? na * (na + 1) / 2 : na * nb;
+            };
+            std::vector<int64_t> kp(PY.n() + 1, 0), kf(PY.n() + 1, 0);
+            for (int j = 0; j < PY.n(); ++j) {
+                kp[j + 1] = kp[j] + PY.ppn[j];
+                kf[j + 1] = kf[j] + upairs(PY, j, l[2], l[3]);
+            }
+            int64_t sq = 0, pq = 0, s64 = 0, cq = 0;
             for (int i = 0; i < nx; ++i) {
                 const int64_t n64 = tc.p64[i + 1] - tc.p64[i];
                 const int64_t n32 = tc.p32[i + 1] - tc.p32[i];
@@ -1060,11 +1071,14 @@ int qbx_jk_stats(qbx_ctx* ctx, qbx_basis* b, double thr, double cut, int64_t* sh
                 sq += cnt;
                 s64 += n64;
                 pq += (int64_t)PX.ppn[i] * kp[cnt];
+                const int64_t fi = upairs(PX, i, l[0], l[1]);
+                if (X_is_Y(cls) && cnt == i + 1)   // the diagonal quartet (bra == ket) counts fi(fi+1)/2
+                    cq += fi * kf[cnt - 1] + fi * (fi + 1) / 2;
+                else
+                    cq += fi * kf[cnt];
             }
-            const int* l = qbx_class_l[cls];
-            const int64_t ncomp = (int64_t)qbx_ncart(l[0]) * qbx_ncart(l[1]) * qbx_ncart(l[2]) * qbx_ncart(l[3]);
             if (shq) shq[cls] = sq;
-            if (cartq) cartq[cls] = sq * ncomp;
+            if (cartq) cartq[cls] = cq;
             if (primq) primq[cls] = pq;
             if (fp64q) fp64q[cls] = s64;
         }

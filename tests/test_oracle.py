@@ -211,3 +211,31 @@ def test_multipole_reduces_to_overlap():
     assert M[1, 0] == pytest.approx(e.overlap()[1, 2], rel=1e-13)
     M0 = e.ob.multipole(c1["center"], (0, 0, 0))
     np.testing.assert_array_equal(M0, e.overlap())
+
+
+def test_reference_orientation_loses_digits_stable_one_does_not():
+    """The reference evaluates each primitive quartet in its canonical index
+    order; for mixed tight/diffuse d quartets the electron-transfer step then
+    loses ~1e-9 absolute.  The oracle's default orientation (larger exponent
+    first in each pair, smaller exponent sum on electron 1) agrees with a
+    McMurchie-Davidson evaluation (40-digit Boys values) to ~1e-13."""
+    import itertools
+    from helpers import random_spd_basis
+    coords, bs = random_spd_basis(0, n_atoms=3)
+    f = qb.flatten_basis(bs)
+    ob = oracle.Basis(f.as_dict())
+
+    def terms(k):
+        return [(tuple(f.prim_cen[f.bf_prim[s]]), f.prim_xpn[f.bf_prim[s]], tuple(f.prim_ang[f.bf_prim[s]]), f.bf_w[s])
+                for s in range(f.bf_ptr[k], f.bf_ptr[k + 1])]
+    q = (9, 9, 9, 26)
+    ref = sum(np.prod([x[3] for x in t]) * md.eri(*[v for x in t for v in x[:3]])
+              for t in itertools.product(*[terms(k) for k in q]))
+    try:
+        oracle.set_reference_orientation(True)
+        v_ref = ob.eri(*q)
+    finally:
+        oracle.set_reference_orientation(False)
+    v = ob.eri(*q)
+    assert abs(v - ref) < 1e-13
+    assert abs(v_ref - ref) > 1e-10
