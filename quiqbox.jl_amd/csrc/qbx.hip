@@ -94,6 +94,9 @@ template <typename T> struct DevBuf {
 };
 
 constexpr double kPi = 3.14159265358979323846;
+constexpr int kStreams = 4;
+// primitive pairs with |c_a c_b exp(-ab/p R^2) / p| below this are dropped
+constexpr double QBX_PRIM_PAIR_CUT = 1e-30;
 
 }  // namespace
 
@@ -103,6 +106,8 @@ constexpr double kPi = 3.14159265358979323846;
 struct qbx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t aux[kStreams] = {};   // concurrent ERI-class launches (default priority)
+    hipEvent_t ev_fork = nullptr, ev_join[kStreams] = {};
     DevBuf<double> boys, bexp;
     bool timing = false;
     std::vector<float> last_ms;
@@ -619,7 +624,21 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
         for (auto& e : ev) QBX_HIP(hipEventCreate(&e));
         QBX_HIP(hipEventRecord(ev[2 * QBX_N_CLASSES], st));
     }
-    for (int cls = 0; cls < QBX_N_CLASSES; ++cls) {
+    // Without timing, the class kernels run concurrently on the auxiliary
+    // streams (largest work first); they only meet in the J/K atomics.
+    const bool fork = !ctx->timing;
+    if (fork) {
+        QBX_HIP(hipEventRecord(ctx->ev_fork, st));
+        for (int k = 0; k < kStreams; ++k) QBX_HIP(hipStreamWaitEvent(ctx->aux[k], ctx->ev_fork, 0));
+    }
+    std::vector<int> order(QBX_N_CLASSES);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return b->t64[x].total + b->t32[x].total > b->t64[y].total + b->t32[y].total;
+    });
+    for (int oi = 0; oi < QBX_N_CLASSES; ++oi) {
+        const int cls = order[oi];
+        hipStream_t ls = fork ? ctx->aux[oi % kStreams] : st;
         if (ctx->timing) QBX_HIP(hipEventRecord(ev[2 * cls], st));
         EriArgs a = base_args(b, cls);
         a.D = dD;
@@ -631,16 +650,22 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
         a.koff = t64.koff.p;
         a.total = t64.total;
         a.nbra = t64.nbra;
-        k_jk64[cls](a, st);
+        k_jk64[cls](a, ls);
         if (mode == QBX_PREC_MIXED) {
             const TaskSet& t32 = b->t32[cls];
             a.prefix = t32.prefix.p;
             a.koff = t32.koff.p;
             a.total = t32.total;
             a.nbra = t32.nbra;
-            k_jk32[cls](a, st);
+            k_jk32[cls](a, ls);
         }
         if (ctx->timing) QBX_HIP(hipEventRecord(ev[2 * cls + 1], st));
+    }
+    if (fork) {
+        for (int k = 0; k < kStreams; ++k) {
+            QBX_HIP(hipEventRecord(ctx->ev_join[k], ctx->aux[k]));
+            QBX_HIP(hipStreamWaitEvent(st, ctx->ev_join[k], 0));
+        }
     }
     const size_t tot = NN * nset;
     hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
@@ -689,6 +714,11 @@ int qbx_init(int32_t device, qbx_ctx** out)
         c->device = device;
         QBX_HIP(hipSetDevice(device));
         QBX_HIP(hipStreamCreate(&c->stream));
+        for (int k = 0; k < kStreams; ++k) {
+            QBX_HIP(hipStreamCreate(&c->aux[k]));
+            QBX_HIP(hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming));
+        }
+        QBX_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         std::vector<double> tab, et;
         build_boys_table(tab, et);
         c->boys.upload(tab, c->stream);
@@ -705,6 +735,12 @@ void qbx_finalize(qbx_ctx* ctx)
     (void)hipStreamSynchronize(ctx->stream);
     ctx->boys.release();
     ctx->bexp.release();
+    for (int k = 0; k < kStreams; ++k) {
+        (void)hipStreamSynchronize(ctx->aux[k]);
+        (void)hipStreamDestroy(ctx->aux[k]);
+        (void)hipEventDestroy(ctx->ev_join[k]);
+    }
+    (void)hipEventDestroy(ctx->ev_fork);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -826,10 +862,17 @@ int qbx_basis_create(qbx_ctx* ctx, int32_t n_prim, const double* prim_cen, const
                 P.shA.push_back(A);
                 P.shB.push_back(B);
                 P.ppo.push_back((int)b->pp.size());
+                int kept = 0;
                 for (int u = 0; u < HA.nprim; ++u)
                     for (int v = 0; v < HB.nprim; ++v) {
                         const double a = b->pexp[HA.poff + u], bb = b->pexp[HB.poff + v];
                         const double p = a + bb;
+                        const double kp = b->pcoef[HA.poff + u] * b->pcoef[HB.poff + v] * std::exp(-a * bb / p * r2) / p;
+                        // primitive pairs whose c_a c_b exp(-ab/p R^2) / p is below 1e-30 cannot
+                        // change any integral above 1e-18 (the other pair's factor is at most
+                        // O(1e2), the Boys value at most 1, polynomial factors below 1e8)
+                        if (std::fabs(kp) < QBX_PRIM_PAIR_CUT) continue;
+                        ++kept;
                         PrimPair q;
                         q.p = p;
                         q.inv2p = 0.5 / p;
@@ -839,11 +882,11 @@ int qbx_basis_create(qbx_ctx* ctx, int32_t n_prim, const double* prim_cen, const
                         q.PAx = q.Px - HA.c[0];
                         q.PAy = q.Py - HA.c[1];
                         q.PAz = q.Pz - HA.c[2];
-                        q.Kp = b->pcoef[HA.poff + u] * b->pcoef[HB.poff + v] * std::exp(-a * bb / p * r2) / p;
+                        q.Kp = kp;
                         q.pad = bb;
                         b->pp.push_back(q);
                     }
-                P.ppn.push_back(HA.nprim * HB.nprim);
+                P.ppn.push_back(kept);
                 for (int t = 0; t < 3; ++t) P.AB.push_back(AB[t]);
                 P.Q.push_back(0.0);
                 b->p1.push_back(make_int2(cl, P.n() - 1));
