@@ -118,6 +118,14 @@ int  qbx_last_timing(qbx_ctx* ctx, float* ms_per_class, float* ms_total);
    under QBX_PREC_MIXED (default 1e-5). */
 int  qbx_set_mixed_cut(qbx_ctx* ctx, double cut);
 
+/* Density screening for qbx_jk / qbx_jk_device (0 = off, the default): shell
+   quartets with Q_ab Q_cd max|D| < thr are skipped, where max|D| is the largest
+   element of the six density blocks (ab, cd, ac, ad, bc, bd) over all DJ/DK sets
+   of the call (Haser & Ahlrichs, J. Comput. Chem. 10, 104 (1989)).  Meant for
+   incremental builds G(D_n) = G(D_{n-1}) + G(D_n - D_{n-1}) in an SCF loop.
+   No reference counterpart: the reference contracts a stored tensor. */
+int  qbx_set_density_screening(qbx_ctx* ctx, double thr);
+
 /* Last error message of the calling thread; returns its full length. */
 int  qbx_last_error(char* buf, size_t len);
 

@@ -246,6 +246,7 @@ def main(outdir):
     for (la, lb, lc, ld) in classes:
         code, fv, fh, nacc = gen_class(la, lb, lc, ld)
         tag = "%d%d%d%d" % (la, lb, lc, ld)
+        flops[tag] = {"vrr_per_prim_quartet": fv, "hrr_per_quartet": fh, "n_acc": nacc}
         variants = [("double", "MODE_JK", "jk64"), ("float", "MODE_JK", "jk32"), ("double", "MODE_DENSE", "dense")]
         if (la, lb) == (lc, ld):
             variants.append(("double", "MODE_SCHWARZ", "schwarz"))

@@ -113,6 +113,7 @@ struct qbx_ctx {
     std::vector<float> last_ms;
     float last_total = 0.f;
     double mixed_cut = 1e-5;
+    double density_thr = 0.0;      // > 0: density-screened J/K
     std::mutex mu;
 };
 
@@ -161,6 +162,7 @@ struct qbx_basis {
     TaskSet t64[QBX_N_CLASSES], t32[QBX_N_CLASSES];
     // work buffers
     DevBuf<double> d_D, d_J, d_K, d_G;
+    DevBuf<double> d_pdmax;        // shell-pair max |D| for density screening
 };
 
 namespace {
@@ -327,6 +329,26 @@ __global__ __launch_bounds__(256) void one_e_kernel(OneArgs a)
     a.out[(size_t)ib * a.N + ia] = acc;
 }
 
+// max over the density blocks of shell pair (s1, s2): |D[s][i][j]|, all 2*nset
+// matrices (DJ and DK sets), i in s1, j in s2
+__global__ void pair_dmax_kernel(const DevShell* sh, int ns, const double* D, int nmat, int N, double* out)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ns * ns) return;
+    const int s1 = g / ns, s2 = g % ns;
+    const DevShell A = sh[s1], B = sh[s2];
+    double m = 0.0;
+    for (int ca = 0; ca < qbx_ncart(A.L); ++ca) {
+        if (A.bf[ca] < 0) continue;
+        for (int cb = 0; cb < qbx_ncart(B.L); ++cb) {
+            if (B.bf[cb] < 0) continue;
+            for (int k = 0; k < nmat; ++k)
+                m = fmax(m, fabs(D[(size_t)k * N * N + (size_t)A.bf[ca] * N + B.bf[cb]]));
+        }
+    }
+    out[g] = m;
+}
+
 __global__ void fill_zero_kernel(double* p, size_t n)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -381,6 +403,7 @@ PairClassDev pcdev(const PairClass& c)
     d.ppo = c.d_ppo.p;
     d.ppn = c.d_ppn.p;
     d.AB = c.d_AB.p;
+    d.Q = c.d_Q.p;
     return d;
 }
 
@@ -624,6 +647,15 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
         for (auto& e : ev) QBX_HIP(hipEventCreate(&e));
         QBX_HIP(hipEventRecord(ev[2 * QBX_N_CLASSES], st));
     }
+    const double* pdmax = nullptr;
+    const int ns = (int)b->shells.size();
+    if (ctx->density_thr > 0.0) {
+        b->d_pdmax.ensure((size_t)ns * ns);
+        hipLaunchKernelGGL(pair_dmax_kernel, dim3((ns * ns + 255) / 256), dim3(256), 0, st, b->d_sh.p, ns, dD,
+                           2 * nset, b->nbf, b->d_pdmax.p);
+        QBX_HIP(hipGetLastError());
+        pdmax = b->d_pdmax.p;
+    }
     // Without timing, the class kernels run concurrently on the auxiliary
     // streams (largest work first); they only meet in the J/K atomics.
     const bool fork = !ctx->timing;
@@ -641,6 +673,10 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
         hipStream_t ls = fork ? ctx->aux[oi % kStreams] : st;
         if (ctx->timing) QBX_HIP(hipEventRecord(ev[2 * cls], st));
         EriArgs a = base_args(b, cls);
+        a.pdmax = pdmax;
+        a.scratch_stream = (void*)st;
+        a.nshell = ns;
+        a.dthr = ctx->density_thr;
         a.D = dD;
         a.J = b->d_J.p;
         a.K = b->d_K.p;
@@ -1144,6 +1180,15 @@ int qbx_last_timing(qbx_ctx* ctx, float* ms, float* total)
         if (ms)
             for (int c = 0; c < QBX_N_CLASSES; ++c) ms[c] = ctx->last_ms[c];
         if (total) *total = ctx->last_total;
+    });
+}
+
+int qbx_set_density_screening(qbx_ctx* ctx, double thr)
+{
+    return guarded([&] {
+        if (!ctx) throw QbxError(QBX_EINVAL, "null context");
+        if (!(thr >= 0.0)) throw QbxError(QBX_EINVAL, "density screening threshold must be >= 0");
+        ctx->density_thr = thr;
     });
 }
 

@@ -40,6 +40,16 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
         ki = a.koff[lo] + (int)(t - a.prefix[lo]);
     }
 
+    if (MODE == MODE_JK && a.pdmax) {
+        // Haser-Ahlrichs bound with the densities of this build
+        const int xA = a.bra.shA[bi], xB = a.bra.shB[bi], xC = a.ket.shA[ki], xD = a.ket.shB[ki];
+        const double* dm = a.pdmax;
+        const int ns = a.nshell;
+        double d = fmax(fmax(dm[xA * ns + xB], dm[xC * ns + xD]), fmax(dm[xA * ns + xC], dm[xA * ns + xD]));
+        d = fmax(d, fmax(dm[xB * ns + xC], dm[xB * ns + xD]));
+        if (a.bra.Q[bi] * a.ket.Q[ki] * d < a.dthr) return;
+    }
+
     R acc[Cls::NACC];
 #pragma unroll
     for (int n = 0; n < Cls::NACC; ++n) acc[n] = (R)0;
@@ -266,9 +276,17 @@ template <int LA, int LB, int LC, int LD, typename R, int MODE>
 void launch_eri(const EriArgs& a, hipStream_t s)
 {
     if (a.total <= 0) return;
+    // Kernels that spill to scratch run on the caller's main stream only: each
+    // extra hardware queue would need its own scratch allocation.
+    static const size_t scratch = [] {
+        hipFuncAttributes fa;
+        if (hipFuncGetAttributes(&fa, (const void*)eri_kernel<LA, LB, LC, LD, R, MODE>) != hipSuccess) return (size_t)1;
+        return (size_t)fa.localSizeBytes;
+    }();
+    hipStream_t use = (scratch > 0 && a.scratch_stream) ? (hipStream_t)a.scratch_stream : s;
     const int threads = 256;
     const int64_t blocks = (a.total + threads - 1) / threads;
-    hipLaunchKernelGGL((eri_kernel<LA, LB, LC, LD, R, MODE>), dim3((unsigned)blocks), dim3(threads), 0, s, a);
+    hipLaunchKernelGGL((eri_kernel<LA, LB, LC, LD, R, MODE>), dim3((unsigned)blocks), dim3(threads), 0, use, a);
     QBX_HIP(hipGetLastError());
 }
 

@@ -14,6 +14,7 @@ struct PairClassDev {
     const int* ppo;
     const int* ppn;
     const double* AB;
+    const double* Q;    // Schwarz bound per pair (sorted descending)
 };
 
 struct EriArgs {
@@ -34,6 +35,12 @@ struct EriArgs {
     int N;
     double* eri;       // dense mode, N^4
     double* qout;      // Schwarz mode, per bra pair
+    // density screening (J/K mode): skip quartets with Q_ab Q_cd max|D| < dthr,
+    // max|D| over the six shell-pair blocks that the quartet reads
+    const double* pdmax;  // [n_shell][n_shell], NULL: off
+    int nshell;
+    double dthr;
+    void* scratch_stream;  // main stream: kernels with scratch are launched there
 };
 
 

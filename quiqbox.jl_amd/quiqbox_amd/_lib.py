@@ -43,6 +43,7 @@ SIGNATURES = {
     "qbx_set_timing": (_i32, [_vp, _i32]),
     "qbx_last_timing": (_i32, [_vp, _pf, _pf]),
     "qbx_set_mixed_cut": (_i32, [_vp, _d]),
+    "qbx_set_density_screening": (_i32, [_vp, _d]),
     "qbx_last_error": (_i32, [ctypes.c_char_p, ctypes.c_size_t]),
 }
 

@@ -45,6 +45,9 @@ class Context:
     def set_mixed_cut(self, cut):
         _lib.check(_lib.load().qbx_set_mixed_cut(self.h, float(cut)))
 
+    def set_density_screening(self, thr):
+        _lib.check(_lib.load().qbx_set_density_screening(self.h, float(thr)))
+
 
 def default_context():
     global _default_ctx
@@ -162,10 +165,10 @@ class GPUBasis:
             G = np.einsum("ai,sab,bj->sij", self.T, G, self.T)
         return G[0] if single else G
 
-    def two_body(self, threshold=1e-12, precision=_lib.QBX_PREC_FP64):
+    def two_body(self, threshold=1e-12, precision=_lib.QBX_PREC_FP64, **kw):
         """The Fock-build operator of this basis (direct, Schwarz-screened)."""
         from .hartree_fock import DirectJK
-        return DirectJK(self, threshold, precision)
+        return DirectJK(self, threshold, precision, **kw)
 
     def jk_device(self, dDJ_ptr, dDK_ptr, dG_ptr, n_set, thr=1e-12, precision=_lib.QBX_PREC_FP64, stream=None):
         if self.T is not None:

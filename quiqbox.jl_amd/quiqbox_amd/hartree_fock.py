@@ -85,19 +85,62 @@ class DenseERI(TwoBodyOperator):
 
 
 class DirectJK(TwoBodyOperator):
-    """Direct, Schwarz-screened J/K on the GPU (qbx_jk)."""
+    """Direct, Schwarz-screened J/K on the GPU (qbx_jk).
 
-    def __init__(self, gpu_basis, threshold=1e-12, precision=_lib.QBX_PREC_FP64):
+    precision: QBX_PREC_FP64, or QBX_PREC_MIXED (FP32 arithmetic for quartets
+        whose Schwarz bound is below the context's FP64 cut).  A mixed operator
+        switches itself to FP64 for good once the SCF reports |dE| below
+        `switch_dE` (the final FP64 Fock rebuilds near convergence).
+    incremental: build G(D) = G(D_ref) + G(D - D_ref) from the last computed
+        pair, with density screening at `threshold` (quartets with
+        Q_ab Q_cd max|D - D_ref| below it are skipped); a full build every
+        `rebuild_every` builds and at the precision switch.
+    """
+
+    def __init__(self, gpu_basis, threshold=1e-12, precision=_lib.QBX_PREC_FP64, incremental=False,
+                 rebuild_every=8, switch_dE=1e-5):
         self.basis = gpu_basis
         self.threshold = threshold
         self.precision = precision
+        self.incremental = incremental
+        self.rebuild_every = rebuild_every
+        self.switch_dE = switch_dE
         self.n_builds = 0
+        self.n_mixed = 0
+        self._ref = None      # (DJ, DK, G) of the last build
+        self._since_full = 0
+
+    def observe(self, dE, dF):
+        """Progress report from the SCF loop (|dE| of the last iteration)."""
+        if self.precision == _lib.QBX_PREC_MIXED and abs(dE) < self.switch_dE:
+            self.precision = _lib.QBX_PREC_FP64
+            self._ref = None   # next build is a full FP64 rebuild
+
+    def _build(self, DJ, DK, density_thr):
+        ctx = self.basis.ctx
+        ctx.set_density_screening(density_thr)
+        try:
+            G = self.basis.jk(DJ, DK, self.threshold, self.precision)
+        finally:
+            ctx.set_density_screening(0.0)
+        self.n_builds += 1
+        if self.precision == _lib.QBX_PREC_MIXED:
+            self.n_mixed += 1
+        return G
 
     def getGcores(self, pairs):
         DJ = np.stack([p[0] for p in pairs])
         DK = np.stack([p[1] for p in pairs])
-        G = self.basis.jk(DJ, DK, self.threshold, self.precision)
-        self.n_builds += 1
+        ref = self._ref
+        if (self.incremental and ref is not None and ref[0].shape == DJ.shape
+                and self._since_full < self.rebuild_every):
+            G = ref[2] + self._build(DJ - ref[0], DK - ref[1], self.threshold)
+            self._since_full += 1
+        else:
+            G = self._build(DJ, DK, 0.0)
+            self._since_full = 0
+        if self.incremental:
+            self._ref = (DJ, DK, G)
         return [G[s] for s in range(len(pairs))]
 
     def getGcore(self, DJ, DK):
@@ -780,6 +823,9 @@ def runHartreeFockCore(HFT, scf, config, C0, maxStep, earlyStop, saveTrace, prin
                 dDrms.append(rmsOf(sh.Dtots[-1] - sh.Dtots[-2]))
                 dFrms.append(getErrorNrms(vars_, S))
             dE, dD, dF = dEs[-1], dDrms[-1], dFrms[-1]
+            observe = getattr(config.twoBody, "observe", None)
+            if observe is not None:
+                observe(dE, dF)
             if printInfo and infoLevel > 0:
                 log("| %d | %.12f | %.3e | %.3e | %.3e" % (i, Etots[-1], dE, dF, dD))
             thr = (bp, dDbreak[l])
@@ -852,7 +898,8 @@ def _is_engine(obj):
     return all(hasattr(obj, k) for k in ("n", "overlap", "core_hamiltonian", "two_body"))
 
 
-def initializeHartreeFock(initCfg, basis, system, twoBody=None, schwarz=1e-12, precision=_lib.QBX_PREC_FP64):
+def initializeHartreeFock(initCfg, basis, system, twoBody=None, schwarz=1e-12, precision=_lib.QBX_PREC_FP64,
+                          incremental=False):
     """Integrals, orthonormalization and initial coefficients (HartreeFock.jl:178-210).
     `basis` is an integral engine (GPUBasis or any object with n, overlap(),
     core_hamiltonian(Z, R), two_body(threshold, precision)) or orbitals."""
@@ -862,7 +909,8 @@ def initializeHartreeFock(initCfg, basis, system, twoBody=None, schwarz=1e-12, p
     Z, R = nuc.charges, nuc.coord_array
     Hcore = gb.core_hamiltonian(Z, R)
     if twoBody is None:
-        twoBody = gb.two_body(schwarz, precision)
+        twoBody = gb.two_body(schwarz, precision, incremental=incremental) if incremental else \
+            gb.two_body(schwarz, precision)
     X = getOrthonormalization(S)
     config = ElecHamiltonianConfig(spin, X, S, Hcore, twoBody)
     HFT = initCfg.HFT
@@ -882,10 +930,14 @@ def initializeHartreeFock(initCfg, basis, system, twoBody=None, schwarz=1e-12, p
 
 
 def runHartreeFock(nucInfo, basis, config=None, *, printInfo=True, infoLevel=defaultHFinfoLevel,
-                   twoBody=None, schwarzThreshold=1e-12, precision=_lib.QBX_PREC_FP64, log=None):
+                   twoBody=None, schwarzThreshold=1e-12, precision=_lib.QBX_PREC_FP64, incremental=False,
+                   log=None):
     """runHartreeFock(nucInfo, basis, config) or runHartreeFock((spin, nucInfo), basis, config)
     (HartreeFock.jl:953-1046).  `basis` is a list of orbitals, a FlatBasis or a
-    GPUBasis.  Returns HFfinalInfo with energy = (E_HF electronic, E_nuclear)."""
+    GPUBasis.  Returns HFfinalInfo with energy = (E_HF electronic, E_nuclear).
+    Beyond the reference: `precision` (QBX_PREC_FP64 / QBX_PREC_MIXED) and
+    `incremental` (density-screened incremental Fock builds) select the GPU
+    Fock-build mode; the defaults reproduce the reference's full FP64 builds."""
     t0 = time.time()
     if isinstance(nucInfo, tuple) and len(nucInfo) == 2 and isinstance(nucInfo[1], NuclearCluster):
         spin, nuc = nucInfo
@@ -910,7 +962,8 @@ def runHartreeFock(nucInfo, basis, config=None, *, printInfo=True, infoLevel=def
         nb = len(basis)
     if nb < max(spin.layout):
         raise ValueError("The basis-set size should be no less than %d." % max(spin.layout))
-    hamil, C0, gb = initializeHartreeFock(config.initial, basis, system, twoBody, schwarzThreshold, precision)
+    hamil, C0, gb = initializeHartreeFock(config.initial, basis, system, twoBody, schwarzThreshold, precision,
+                                          incremental)
     t1 = time.time()
     if printInfo and infoLevel > 1:
         (log or print)("Hartree–Fock (HF) Initialization:\n•HF Type: %s\n•Basis Set Size: %d\n"
@@ -925,7 +978,8 @@ def runHartreeFock(nucInfo, basis, config=None, *, printInfo=True, infoLevel=def
     Eel = vars_[0].shared.Etots[-1]
     res = HFfinalInfo(system, (Eel, Enuc), tuple(tv.Cs[-1] for tv in vars_), tuple(tv.Ds[-1] for tv in vars_),
                       Fs, occu, vars_, converged, gb,
-                      dict(init_s=t1 - t0, scf_s=t2 - t1, fock_builds=getattr(hamil.twoBody, "n_builds", None)))
+                      dict(init_s=t1 - t0, scf_s=t2 - t1, fock_builds=getattr(hamil.twoBody, "n_builds", None),
+                           mixed_builds=getattr(hamil.twoBody, "n_mixed", None)))
     if printInfo:
         (log or print)("Hartree–Fock Energy ¦ Nuclear Repulsion ¦ Total Energy\n%.10f Ha   %.10f Ha   %.10f Ha"
                        % (Eel, Enuc, Eel + Enuc))

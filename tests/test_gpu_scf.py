@@ -76,3 +76,24 @@ def test_config2_small_cluster_matches_oracle():
     ref = qb.runHartreeFock(nuc, OracleEngine(bs), cfg, printInfo=False)
     _say("  E_gpu %.12f E_oracle %.12f conv %s %s" % (res.energy[0], ref.energy[0], res.converged, ref.converged))
     assert abs(res.energy[0] - ref.energy[0]) < 1e-9
+
+
+def test_incremental_and_mixed_scf_match_plain_fp64():
+    """Density-screened incremental Fock builds and mixed precision (FP32
+    quartets below the cut, final FP64 rebuilds) converge to the plain FP64
+    energy of a 24-atom cluster (96 functions)."""
+    nuc, bs = random_cluster(24, 5, box=12.0)
+    gb = qb.GPUBasis(bs)
+    cfg = qb.HFconfig(initial="CoreH", strategy=qb.SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-10)), maxStep=150)
+    _say("24-atom cluster plain FP64 n_bf=%d" % gb.n)
+    ref = qb.runHartreeFock(nuc, gb, cfg, printInfo=False)
+    _say("  incremental FP64")
+    inc = qb.runHartreeFock(nuc, gb, cfg, printInfo=False, incremental=True)
+    _say("  mixed + incremental")
+    mix = qb.runHartreeFock(nuc, gb, cfg, printInfo=False, incremental=True, precision=qb.QBX_PREC_MIXED)
+    _say("  E plain %.12f incremental %.12f mixed %.12f (mixed builds %s of %s)" % (
+        ref.energy[0], inc.energy[0], mix.energy[0], mix.timing["mixed_builds"], mix.timing["fock_builds"]))
+    assert ref.converged and inc.converged and mix.converged
+    assert abs(inc.energy[0] - ref.energy[0]) < 1e-9
+    assert abs(mix.energy[0] - ref.energy[0]) < 1e-8
+    assert mix.timing["mixed_builds"] > 0

@@ -145,3 +145,41 @@ def test_h10_chain_oracle_energy():
     # variational sanity: above the sum of H-atom energies of a larger basis is not asserted;
     # the value is pinned against the GPU path in test_gpu_scf.py
     assert -12.0 < sum(res.energy) < -4.0
+
+
+class _DenseJKBasis:
+    """Stand-in for GPUBasis.jk built on the oracle's dense tensor (host logic test)."""
+
+    class _Ctx:
+        def __init__(self):
+            self.density_thr = []
+
+        def set_density_screening(self, thr):
+            self.density_thr.append(thr)
+
+    def __init__(self, eng):
+        self.eng = eng
+        self.ctx = self._Ctx()
+        self.calls = []
+
+    def jk(self, DJ, DK, thr, precision):
+        import oracle
+        self.calls.append(precision)
+        E = self.eng.eri()
+        return np.stack([oracle.gcore(E, dj, dk) for dj, dk in zip(DJ, DK)])
+
+
+def test_incremental_and_mixed_switch_host_logic():
+    from quiqbox_amd.hartree_fock import DirectJK
+    nuc, eng = hho()
+    cfg = qb.HFconfig(initial="CoreH", strategy=qb.SCFconfig(("DIIS",), (1e-10,)))
+    ref = qb.runHartreeFock(nuc, eng, cfg, printInfo=False)
+    fake = _DenseJKBasis(eng)
+    op = DirectJK(fake, 1e-12, qb.QBX_PREC_MIXED, incremental=True, rebuild_every=3)
+    res = qb.runHartreeFock(nuc, eng, cfg, printInfo=False, twoBody=op)
+    assert res.energy[0] == pytest.approx(ref.energy[0], abs=1e-10)
+    # incremental builds set density screening, full builds clear it
+    assert any(t > 0 for t in fake.ctx.density_thr) and 0.0 in fake.ctx.density_thr
+    # precision switched from mixed to FP64 once |dE| fell below the switch threshold
+    assert fake.calls[0] == qb.QBX_PREC_MIXED and fake.calls[-1] == qb.QBX_PREC_FP64
+    assert op.n_mixed > 0 and op.n_builds > op.n_mixed
