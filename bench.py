@@ -15,6 +15,9 @@ oracle (a restatement of the reference's per-component Obara-Saika
 algorithm) on a bounded sample of the same quartets.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-scf] [--no-cpu]
+       python bench.py --copy-calibration   (a plain device copy of known size,
+       profiled with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to scale the
+       counters to bytes)
 """
 import argparse
 import ctypes
@@ -71,6 +74,34 @@ class Hip:
 
     def free(self, p):
         self.L.hipFree(ctypes.c_void_p(p))
+
+
+def copy_calibration(n_bytes=1 << 30, reps=3):
+    """A plain elementwise device copy (y = -x over n_bytes of doubles): reads
+    and writes exactly n_bytes per launch, the yardstick for the PMC counters."""
+    import torch
+    x = torch.ones(n_bytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": n_bytes, "bytes_written": n_bytes, "launches": reps,
+                                           "kernel": "elementwise neg (float64)"}}), flush=True)
+
+
+def traffic_from_profile(tag):
+    """HBM bytes per launch of the class-`tag` JK kernel from the committed
+    PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy calibration), or
+    None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "eri_jk64_hbm_traffic_592bf.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if len(w) >= 3 and w[0] == "traffic" and w[1] == tag:
+                return float(w[2])
+    return None
 
 
 def core_guess_density(gb, nuc, nocc):
@@ -145,7 +176,11 @@ def main():
     ap.add_argument("--no-scf", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -230,7 +265,7 @@ def main():
                    "primitive_shell_quartets": int(stats["prim_quartets"].sum())},
         "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "eri_kernel<%s, double, JK>" % tag,
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profile(tag),
                      "kernel_ms": float(per_class[dom]), "all_eri_kernels_tflops": total_tflops},
     }
 

@@ -5,8 +5,9 @@ For every class (la lb | lc ld) with la >= lb, lc >= ld and
 (la, lb) >= (lc, ld) this emits a straight-line device function
 
     template <typename R> struct EriClass<la, lb, lc, ld, R> {
-        static __device__ void contract(...);   // primitive loop + VRR
-        static __device__ void transfer(...);   // HRR, ket then bra
+        static __device__ void vrr(...);       // per primitive quartet, accumulates [e0|f0]
+        static __device__ void hrr_ket(...);   // [e0|f0] -> [e|cd)
+        static __device__ void hrr_bra(...);   // [e|cd) -> (ab|cd) for one component a
     };
 
 Scheme (Head-Gordon & Pople, J. Chem. Phys. 89, 5777 (1988), built on the
@@ -19,7 +20,10 @@ Obara-Saika recurrences, J. Chem. Phys. 84, 3963 (1986)):
   3. [e0|f0] is summed over the primitive quartets of the shell quartet
      (contraction happens before any horizontal transfer);
   4. the horizontal recurrence moves angular momentum to the second centre
-     of the ket, then of the bra: (a,b+1_i| = (a+1_i,b| + (A-B)_i (a,b|.
+     of the ket (hrr_ket, once), then of the bra (hrr_bra, one Cartesian
+     component of shell A at a time, so the caller can consume each slice of
+     the integral block before the next is formed):
+     (a,b+1_i| = (a+1_i,b| + (A-B)_i (a,b|.
 
 Cartesian components follow the order used by Quiqbox (Lexicons.jl:17-36):
 for i = L..0, for j = L-i..0: (i, j, L-i-j).
@@ -168,7 +172,10 @@ def gen_class(la, lb, lc, ld):
     flops_vrr = em.flops + len(targets)
     vrr_code = "\n".join("        " + l for l in em.lines + acc_lines)
 
-    # ---- HRR: ket (c,d) from f, then bra (a,b) from e ----
+    # ---- HRR, ket: [e | c d) from [e0|f0] for every e; kacc[e * NCD + cd] ----
+    C_, D_ = cart(lc), cart(ld)
+    nC, nD = len(C_), len(D_)
+    NCD = nC * nD
     hm = Emitter()
     ket_memo = {}
 
@@ -190,53 +197,73 @@ def gen_class(la, lb, lc, ld):
         ket_memo[key] = name
         return name
 
-    bra_memo = {}
+    kouts = []
+    for ie, e in enumerate(es):
+        for ic, c in enumerate(C_):
+            for idd, d in enumerate(D_):
+                kouts.append("kacc[%d] = %s;" % (ie * NCD + ic * nD + idd, K(ie, c, d)))
+    ket_code = "\n".join("        " + l for l in hm.lines + kouts)
+    flops_hrr = hm.flops
 
-    def B(a, b, c, d):
-        key = (a, b, c, d)
-        if key in bra_memo:
-            return bra_memo[key]
-        if sum(b) == 0:
-            r = K(eidx[a], c, d)
-            bra_memo[key] = r
-            return r
-        i = pick_axis(b)
-        b1 = sub1(b, i)
-        x = B(add1(a, i), b1, c, d)
-        y = B(a, b1, c, d)
-        name = hm.fresh("h")
-        hm.lines.append("const R %s = %s + AB%s * %s;" % (name, x, AX[i], y))
-        hm.flops += 2
-        bra_memo[key] = name
-        return name
+    # ---- HRR, bra, one Cartesian component a of shell A at a time:
+    #      out[ib * NCD + cd] = (a b | cd) from kacc ----
+    A_, B_ = cart(la), cart(lb)
+    nB = len(B_)
+    cases = []
+    for ia, a0 in enumerate(A_):
+        bm = Emitter()
+        memo = {}
 
-    outs = []
-    A_, B_, C_, D_ = cart(la), cart(lb), cart(lc), cart(ld)
-    nB, nC, nD = len(B_), len(C_), len(D_)
-    for ia, a in enumerate(A_):
-        for ib, b in enumerate(B_):
-            for ic, c in enumerate(C_):
-                for idd, d in enumerate(D_):
-                    r = B(a, b, c, d)
-                    outs.append("out[%d] = %s;" % (((ia * nB + ib) * nC + ic) * nD + idd, r))
-    hrr_code = "\n".join("        " + l for l in hm.lines + outs)
+        def Bv(a, b, cd):
+            key = (a, b, cd)
+            if key in memo:
+                return memo[key]
+            if sum(b) == 0:
+                r = "kacc[%d]" % (eidx[a] * NCD + cd)
+                memo[key] = r
+                return r
+            i = pick_axis(b)
+            b1 = sub1(b, i)
+            x = Bv(add1(a, i), b1, cd)
+            y = Bv(a, b1, cd)
+            name = bm.fresh("h")
+            bm.lines.append("const R %s = %s + AB%s * %s;" % (name, x, AX[i], y))
+            bm.flops += 2
+            memo[key] = name
+            return name
+
+        outs = []
+        for ib, b0 in enumerate(B_):
+            for cd in range(NCD):
+                outs.append("out[%d] = %s;" % (ib * NCD + cd, Bv(a0, b0, cd)))
+        flops_hrr += bm.flops
+        body = "\n".join("            " + l for l in bm.lines + outs)
+        cases.append("        case %d: {\n%s\n            break;\n        }" % (ia, body))
+    bra_code = "        switch (ia) {\n" + "\n".join(cases) + "\n        default: break;\n        }"
 
     tag = "%d%d%d%d" % (la, lb, lc, ld)
     code = []
     code.append("template <typename R> struct EriClass<%d, %d, %d, %d, R> {" % (la, lb, lc, ld))
     code.append("    static constexpr int NACC = %d;" % (nE * nF))
+    code.append("    static constexpr int NE = %d;" % nE)
     code.append("    static constexpr int LT = %d;" % Lt)
     code.append("    static __device__ __forceinline__ void vrr(const R* __restrict__ F, R* __restrict__ acc,")
     code.append("        R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz,")
     code.append("        R WQx, R WQy, R WQz, R inv2p, R inv2q, R inv2pq, R rho_p, R rho_q) {")
     code.append(vrr_code)
     code.append("    }")
-    code.append("    static __device__ __forceinline__ void hrr(const R* __restrict__ acc, R* __restrict__ out,")
-    code.append("        R ABx, R ABy, R ABz, R CDx, R CDy, R CDz) {")
-    code.append(hrr_code)
+    code.append("    // kacc[e * NCD + cd] = [e | c d)")
+    code.append("    static __device__ __forceinline__ void hrr_ket(const R* __restrict__ acc, R* __restrict__ kacc,")
+    code.append("        R CDx, R CDy, R CDz) {")
+    code.append(ket_code)
+    code.append("    }")
+    code.append("    // out[ib * NCD + cd] = (a b | c d) for Cartesian component ia of shell A")
+    code.append("    static __device__ __forceinline__ void hrr_bra(int ia, const R* __restrict__ kacc, R* __restrict__ out,")
+    code.append("        R ABx, R ABy, R ABz) {")
+    code.append(bra_code)
     code.append("    }")
     code.append("};")
-    return "\n".join(code), flops_vrr, hm.flops, nE * nF
+    return "\n".join(code), flops_vrr, flops_hrr, nE * nF
 
 
 def main(outdir):

@@ -82,10 +82,12 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
         }
     }
 
+    constexpr int NCD = NC * ND;
     const double* ABp = a.bra.AB + 3 * bi;
     const double* CDp = a.ket.AB + 3 * ki;
-    R outR[NA * NB * NC * ND];
-    Cls::hrr(acc, outR, (R)ABp[0], (R)ABp[1], (R)ABp[2], (R)CDp[0], (R)CDp[1], (R)CDp[2]);
+    R kacc[Cls::NE * NCD];
+    Cls::hrr_ket(acc, kacc, (R)CDp[0], (R)CDp[1], (R)CDp[2]);
+    const R ABx = (R)ABp[0], ABy = (R)ABp[1], ABz = (R)ABp[2];
 
     const int sA = a.bra.shA[bi], sB = a.bra.shB[bi];
     const int sC = a.ket.shA[ki], sD = a.ket.shB[ki];
@@ -93,33 +95,6 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
     const DevShell& SB = a.sh[sB];
     const DevShell& SC = a.sh[sC];
     const DevShell& SD = a.sh[sD];
-
-    double I[NA * NB * NC * ND];
-#pragma unroll
-    for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-        for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-            for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-                for (int id = 0; id < ND; ++id) {
-                    const int n = ((ia * NB + ib) * NC + ic) * ND + id;
-                    I[n] = (double)outR[n] * (SA.sc[ia] * SB.sc[ib]) * (SC.sc[ic] * SD.sc[id]);
-                }
-
-    if (MODE == MODE_SCHWARZ) {
-        double mx = 0.0;
-#pragma unroll
-        for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-            for (int ib = 0; ib < NB; ++ib) {
-                const double v = fabs(I[((ia * NB + ib) * NC + ia) * ND + ib]);
-                mx = v > mx ? v : mx;
-            }
-        a.qout[bi] = sqrt(mx);
-        return;
-    }
-
     int fa[NA], fb[NB], fc[NC], fd[ND];
 #pragma unroll
     for (int n = 0; n < NA; ++n) fa[n] = SA.bf[n];
@@ -129,20 +104,47 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
     for (int n = 0; n < NC; ++n) fc[n] = SC.bf[n];
 #pragma unroll
     for (int n = 0; n < ND; ++n) fd[n] = SD.bf[n];
+    // product of the ket component scales (weight ratios inside a shell)
+    double scd[NCD];
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+        for (int id = 0; id < ND; ++id) scd[ic * ND + id] = SC.sc[ic] * SD.sc[id];
+
+    if (MODE == MODE_SCHWARZ) {
+        // bra == ket: the diagonal (ab|ab) elements of each slice a
+        double mx = 0.0;
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia) {
+            R out[NB * NCD];
+            Cls::hrr_bra(ia, kacc, out, ABx, ABy, ABz);
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib) {
+                const double w = SA.sc[ia] * SB.sc[ib];
+                const double v = fabs((double)out[ib * NCD + ia * ND + ib] * w * w);
+                mx = v > mx ? v : mx;
+            }
+        }
+        a.qout[bi] = sqrt(mx);
+        return;
+    }
 
     const size_t N = (size_t)a.N;
     if (MODE == MODE_DENSE) {
 #pragma unroll
-        for (int ia = 0; ia < NA; ++ia)
+        for (int ia = 0; ia < NA; ++ia) {
+            R out[NB * NCD];
+            Cls::hrr_bra(ia, kacc, out, ABx, ABy, ABz);
 #pragma unroll
             for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
                 for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
                     for (int id = 0; id < ND; ++id) {
-                        const size_t i = fa[ia], j = fb[ib], k = fc[ic], l = fd[id];
                         if (fa[ia] < 0 || fb[ib] < 0 || fc[ic] < 0 || fd[id] < 0) continue;
-                        const double v = I[((ia * NB + ib) * NC + ic) * ND + id];
+                        const size_t i = fa[ia], j = fb[ib], k = fc[ic], l = fd[id];
+                        const double v = (double)out[ib * NCD + ic * ND + id] * (SA.sc[ia] * SB.sc[ib]) *
+                                         scd[ic * ND + id];
                         a.eri[((i * N + j) * N + k) * N + l] = v;
                         a.eri[((j * N + i) * N + k) * N + l] = v;
                         a.eri[((i * N + j) * N + l) * N + k] = v;
@@ -152,16 +154,18 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
                         a.eri[((k * N + l) * N + j) * N + i] = v;
                         a.eri[((l * N + k) * N + j) * N + i] = v;
                     }
+        }
         return;
     }
 
-    // ---- MODE_JK: digest with the degeneracy of the unique shell quartet ----
+    // ---- MODE_JK: digest, one bra component a at a time, with the
+    //      degeneracy of the unique shell quartet ----
     double f = 1.0;
     if (sA == sB) f *= 0.5;
     if (sC == sD) f *= 0.5;
     if (a.same && bi == ki) f *= 0.5;
     const double fj = 4.0 * f, fk = 2.0 * f;
-    // missing components carry weight 0 and point at function 0
+    // missing components carry scale 0 and point at function 0
 #pragma unroll
     for (int n = 0; n < NA; ++n) fa[n] = fa[n] < 0 ? 0 : fa[n];
 #pragma unroll
@@ -176,94 +180,70 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
         const double* __restrict__ DK = a.D + (size_t)(a.nset + s) * N * N;
         double* Js = a.J + (size_t)s * N * N;
         double* Ks = a.K + (size_t)s * N * N;
-        double dab[NA * NB], dcd[NC * ND];
-#pragma unroll
-        for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-            for (int ib = 0; ib < NB; ++ib) dab[ia * NB + ib] = DJ[fa[ia] * N + fb[ib]];
+        // blocks that every slice reads
+        double dcd[NCD], dbc[NB * NC], dbd[NB * ND];
 #pragma unroll
         for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
             for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = DJ[fc[ic] * N + fd[id]];
-        // J_ab += 4f sum_cd I D_cd ;  J_cd += 4f sum_ab I D_ab
-        double jcd[NC * ND];
-#pragma unroll
-        for (int n = 0; n < NC * ND; ++n) jcd[n] = 0.0;
-#pragma unroll
-        for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-            for (int ib = 0; ib < NB; ++ib) {
-                double jab = 0.0;
-                const double d = dab[ia * NB + ib];
-#pragma unroll
-                for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-                    for (int id = 0; id < ND; ++id) {
-                        const double v = I[((ia * NB + ib) * NC + ic) * ND + id];
-                        jab += v * dcd[ic * ND + id];
-                        jcd[ic * ND + id] += v * d;
-                    }
-                atomicAdd(Js + fa[ia] * N + fb[ib], fj * jab);
-            }
-#pragma unroll
-        for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-            for (int id = 0; id < ND; ++id) atomicAdd(Js + fc[ic] * N + fd[id], fj * jcd[ic * ND + id]);
-
-        // K_ad += 2f sum_bc I D_bc ; K_bd += 2f sum_ac I D_ac ;
-        // K_ac += 2f sum_bd I D_bd ; K_bc += 2f sum_ad I D_ad
-        double dbc[NB * NC], dac[NA * NC], dbd[NB * ND], dad[NA * ND];
 #pragma unroll
         for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
             for (int ic = 0; ic < NC; ++ic) dbc[ib * NC + ic] = DK[fb[ib] * N + fc[ic]];
 #pragma unroll
-        for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-            for (int ic = 0; ic < NC; ++ic) dac[ia * NC + ic] = DK[fa[ia] * N + fc[ic]];
-#pragma unroll
         for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
             for (int id = 0; id < ND; ++id) dbd[ib * ND + id] = DK[fb[ib] * N + fd[id]];
+        // sums over all slices
+        double jcd[NCD], kbd[NB * ND], kbc[NB * NC];
 #pragma unroll
-        for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-            for (int id = 0; id < ND; ++id) dad[ia * ND + id] = DK[fa[ia] * N + fd[id]];
-        double kad[NA * ND], kbd[NB * ND], kac[NA * NC], kbc[NB * NC];
-#pragma unroll
-        for (int n = 0; n < NA * ND; ++n) kad[n] = 0.0;
+        for (int n = 0; n < NCD; ++n) jcd[n] = 0.0;
 #pragma unroll
         for (int n = 0; n < NB * ND; ++n) kbd[n] = 0.0;
 #pragma unroll
-        for (int n = 0; n < NA * NC; ++n) kac[n] = 0.0;
-#pragma unroll
         for (int n = 0; n < NB * NC; ++n) kbc[n] = 0.0;
 #pragma unroll
-        for (int ia = 0; ia < NA; ++ia)
+        for (int ia = 0; ia < NA; ++ia) {
+            R out[NB * NCD];
+            Cls::hrr_bra(ia, kacc, out, ABx, ABy, ABz);
+            double dac[NC], dad[ND], kac[NC], kad[ND];
 #pragma unroll
-            for (int ib = 0; ib < NB; ++ib)
+            for (int ic = 0; ic < NC; ++ic) { dac[ic] = DK[fa[ia] * N + fc[ic]]; kac[ic] = 0.0; }
+#pragma unroll
+            for (int id = 0; id < ND; ++id) { dad[id] = DK[fa[ia] * N + fd[id]]; kad[id] = 0.0; }
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib) {
+                const double sab = SA.sc[ia] * SB.sc[ib];
+                const double dab = DJ[fa[ia] * N + fb[ib]];
+                double jab = 0.0;
 #pragma unroll
                 for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
                     for (int id = 0; id < ND; ++id) {
-                        const double v = I[((ia * NB + ib) * NC + ic) * ND + id];
-                        kad[ia * ND + id] += v * dbc[ib * NC + ic];
-                        kbd[ib * ND + id] += v * dac[ia * NC + ic];
-                        kac[ia * NC + ic] += v * dbd[ib * ND + id];
-                        kbc[ib * NC + ic] += v * dad[ia * ND + id];
+                        const int cd = ic * ND + id;
+                        const double v = (double)out[ib * NCD + cd] * (sab * scd[cd]);
+                        jab += v * dcd[cd];
+                        jcd[cd] += v * dab;
+                        kad[id] += v * dbc[ib * NC + ic];
+                        kac[ic] += v * dbd[ib * ND + id];
+                        kbd[ib * ND + id] += v * dac[ic];
+                        kbc[ib * NC + ic] += v * dad[id];
                     }
+                atomicAdd(Js + fa[ia] * N + fb[ib], fj * jab);
+            }
 #pragma unroll
-        for (int ia = 0; ia < NA; ++ia)
+            for (int id = 0; id < ND; ++id) atomicAdd(Ks + fa[ia] * N + fd[id], fk * kad[id]);
 #pragma unroll
-            for (int id = 0; id < ND; ++id) atomicAdd(Ks + fa[ia] * N + fd[id], fk * kad[ia * ND + id]);
+            for (int ic = 0; ic < NC; ++ic) atomicAdd(Ks + fa[ia] * N + fc[ic], fk * kac[ic]);
+        }
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) atomicAdd(Js + fc[ic] * N + fd[id], fj * jcd[ic * ND + id]);
 #pragma unroll
         for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
             for (int id = 0; id < ND; ++id) atomicAdd(Ks + fb[ib] * N + fd[id], fk * kbd[ib * ND + id]);
-#pragma unroll
-        for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-            for (int ic = 0; ic < NC; ++ic) atomicAdd(Ks + fa[ia] * N + fc[ic], fk * kac[ia * NC + ic]);
 #pragma unroll
         for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
