@@ -239,7 +239,7 @@ def main():
     total_tflops = cflops.sum() / (per_class.sum() * 1e-3) / 1e12
     log("dominant kernel: class (%s) %.2f ms, %.2f TFLOP/s; all ERI kernels %.2f ms, %.2f TFLOP/s"
         % (tag, per_class[dom], achieved, per_class.sum(), total_tflops))
-    for c in np.argsort(-per_class)[:8]:
+    for c in np.argsort(-per_class):
         log("  class %s: %.3f ms, %d shell quartets, %.2f TFLOP/s" % (
             "%d%d%d%d" % tuple(classes[c]), per_class[c], stats["shell_quartets"][c],
             cflops[c] / max(per_class[c], 1e-9) / 1e9))

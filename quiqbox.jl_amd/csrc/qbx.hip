@@ -109,6 +109,7 @@ struct qbx_ctx {
     hipStream_t aux[kStreams] = {};   // concurrent ERI-class launches (default priority)
     hipEvent_t ev_fork = nullptr, ev_join[kStreams] = {};
     DevBuf<double> boys, bexp;
+    DevBuf<double> boysl;          // per-order rows for the ERI kernels (LDS-staged)
     bool timing = false;
     std::vector<float> last_ms;
     float last_total = 0.f;
@@ -130,7 +131,9 @@ struct PairClass {
 struct TaskSet {
     DevBuf<int64_t> prefix;
     DevBuf<int> koff;
+    DevBuf<int> cpre;          // waves (chunks of 64 kets) per bra, prefix sums
     int64_t total = 0;
+    int nwave = 0;
     int nbra = 0;
 };
 
@@ -172,10 +175,13 @@ namespace {
 //   F_m(T) = e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1))
 // at the top order, then the stable downward recurrence.
 // ---------------------------------------------------------------------------
-void build_boys_table(std::vector<double>& tab, std::vector<double>& et)
+void build_boys_table(std::vector<double>& tab, std::vector<double>& et, std::vector<double>& rows)
 {
+    static_assert(QBX_BOYSL_LMAX + QBX_BOYS_TAYLOR <= QBX_BOYS_NCOL, "Boys table too narrow");
+    static_assert(QBX_BOYSL_ROWS == QBX_BOYS_ROWS, "Boys row counts differ");
     tab.assign((size_t)QBX_BOYS_ROWS * QBX_BOYS_NCOL, 0.0);
     et.assign(QBX_BOYS_ROWS, 0.0);
+    rows.assign((size_t)(QBX_BOYSL_LMAX + 1) * QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE, 0.0);
     const int mtop = QBX_BOYS_NCOL - 1;
     for (int i = 0; i < QBX_BOYS_ROWS; ++i) {
         long double T = (long double)i * QBX_BOYS_DT;
@@ -191,6 +197,15 @@ void build_boys_table(std::vector<double>& tab, std::vector<double>& et)
         for (int m = mtop - 1; m >= 0; --m) F[m] = (2.0L * T * F[m + 1] + e) / (2 * m + 1);
         for (int m = 0; m < QBX_BOYS_NCOL; ++m) tab[(size_t)i * QBX_BOYS_NCOL + m] = (double)F[m];
         et[i] = (double)e;
+        for (int L = 0; L <= QBX_BOYSL_LMAX; ++L) {
+            double* r = &rows[((size_t)L * QBX_BOYSL_ROWS + i) * QBX_BOYSL_STRIDE];
+            long double fact = 1.0L;
+            for (int k = 0; k < QBX_BOYS_TAYLOR; ++k) {
+                if (k > 0) fact *= k;
+                r[k] = (double)(F[L + k] / fact);
+            }
+            r[QBX_BOYS_TAYLOR] = (double)e;
+        }
     }
 }
 
@@ -434,6 +449,8 @@ EriArgs base_args(qbx_basis* b, int cls)
     a.sh = b->d_sh.p;
     a.boys = b->ctx->boys.p;
     a.bexp = b->ctx->bexp.p;
+    const int lt = qbx_class_l[cls][0] + qbx_class_l[cls][1] + qbx_class_l[cls][2] + qbx_class_l[cls][3];
+    a.boysl = b->ctx->boysl.p + (size_t)lt * QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE;
     a.same = X == Y;
     a.N = b->nbf;
     return a;
@@ -457,6 +474,8 @@ void ensure_schwarz(qbx_basis* b)
         a.sh = b->d_sh.p;
         a.boys = b->ctx->boys.p;
         a.bexp = b->ctx->bexp.p;
+        const int lt = 2 * (qbx_pair_class_l[c][0] + qbx_pair_class_l[c][1]);
+        a.boysl = b->ctx->boysl.p + (size_t)lt * QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE;
         a.total = P.n();
         a.nbra = P.n();
         a.same = 1;
@@ -514,6 +533,7 @@ int count_ge(const std::vector<double>& q, double v)
 struct TaskCounts {
     std::vector<int64_t> p64, p32;
     std::vector<int> k64, k32;
+    std::vector<int> w64, w32;   // waves of 64 kets per bra, prefix sums
 };
 
 void task_counts(const qbx_basis* b, int cls, double thr, int mode, double cut, TaskCounts& tc)
@@ -526,6 +546,8 @@ void task_counts(const qbx_basis* b, int cls, double thr, int mode, double cut, 
     tc.p32.assign(nx + 1, 0);
     tc.k64.assign(std::max(nx, 1), 0);
     tc.k32.assign(std::max(nx, 1), 0);
+    tc.w64.assign(nx + 1, 0);
+    tc.w32.assign(nx + 1, 0);
     for (int i = 0; i < nx; ++i) {
         const double qi = PX.Q[i];
         int cnt;
@@ -541,6 +563,8 @@ void task_counts(const qbx_basis* b, int cls, double thr, int mode, double cut, 
         tc.k32[i] = c64;
         tc.p64[i + 1] = tc.p64[i] + c64;
         tc.p32[i + 1] = tc.p32[i] + (cnt - c64);
+        tc.w64[i + 1] = tc.w64[i] + (c64 + 63) / 64;
+        tc.w32[i + 1] = tc.w32[i] + (cnt - c64 + 63) / 64;
     }
 }
 
@@ -555,11 +579,15 @@ void ensure_tasks(qbx_basis* b, double thr, int mode, double cut)
         const int nx = b->pc[class_pc(cls, 0)].n();
         b->t64[cls].prefix.upload(tc.p64, st);
         b->t64[cls].koff.upload(tc.k64, st);
+        b->t64[cls].cpre.upload(tc.w64, st);
         b->t64[cls].total = tc.p64[nx];
+        b->t64[cls].nwave = tc.w64[nx];
         b->t64[cls].nbra = nx;
         b->t32[cls].prefix.upload(tc.p32, st);
         b->t32[cls].koff.upload(tc.k32, st);
+        b->t32[cls].cpre.upload(tc.w32, st);
         b->t32[cls].total = tc.p32[nx];
+        b->t32[cls].nwave = tc.w32[nx];
         b->t32[cls].nbra = nx;
     }
     QBX_HIP(hipStreamSynchronize(st));
@@ -684,14 +712,18 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
         const TaskSet& t64 = b->t64[cls];
         a.prefix = t64.prefix.p;
         a.koff = t64.koff.p;
+        a.cpre = t64.cpre.p;
         a.total = t64.total;
+        a.nwave = t64.nwave;
         a.nbra = t64.nbra;
         k_jk64[cls](a, ls);
         if (mode == QBX_PREC_MIXED) {
             const TaskSet& t32 = b->t32[cls];
             a.prefix = t32.prefix.p;
             a.koff = t32.koff.p;
+            a.cpre = t32.cpre.p;
             a.total = t32.total;
+            a.nwave = t32.nwave;
             a.nbra = t32.nbra;
             k_jk32[cls](a, ls);
         }
@@ -755,10 +787,11 @@ int qbx_init(int32_t device, qbx_ctx** out)
             QBX_HIP(hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming));
         }
         QBX_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-        std::vector<double> tab, et;
-        build_boys_table(tab, et);
+        std::vector<double> tab, et, rows;
+        build_boys_table(tab, et, rows);
         c->boys.upload(tab, c->stream);
         c->bexp.upload(et, c->stream);
+        c->boysl.upload(rows, c->stream);
         QBX_HIP(hipStreamSynchronize(c->stream));
         *out = c.release();
     });
@@ -771,6 +804,7 @@ void qbx_finalize(qbx_ctx* ctx)
     (void)hipStreamSynchronize(ctx->stream);
     ctx->boys.release();
     ctx->bexp.release();
+    ctx->boysl.release();
     for (int k = 0; k < kStreams; ++k) {
         (void)hipStreamSynchronize(ctx->aux[k]);
         (void)hipStreamDestroy(ctx->aux[k]);
@@ -1047,13 +1081,16 @@ int qbx_eri_dense(qbx_ctx* ctx, const qbx_basis* bc, double* eri)
             const int nx = b->pc[class_pc(cls, 0)].n();
             if (tc.p64[nx] == 0) continue;
             DevBuf<int64_t> pre;
-            DevBuf<int> ko;
+            DevBuf<int> ko, cw;
             pre.upload(tc.p64, st);
             ko.upload(tc.k64, st);
+            cw.upload(tc.w64, st);
             EriArgs a = base_args(b, cls);
             a.prefix = pre.p;
             a.koff = ko.p;
+            a.cpre = cw.p;
             a.total = tc.p64[nx];
+            a.nwave = tc.w64[nx];
             a.nbra = nx;
             a.eri = d.p;
             k_dense[cls](a, st);

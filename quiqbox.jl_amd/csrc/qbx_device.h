@@ -18,6 +18,12 @@
 #define QBX_BOYS_ROWS 602
 #define QBX_BOYS_NCOL 17     // F_0 .. F_16: highest order 8 plus 8 Taylor terms
 #define QBX_BOYS_TAYLOR 8
+// Per-order rows for the ERI kernels (staged in LDS): for total angular
+// momentum L, row i holds F_{L+k}(T_i) / k! for k = 0..7, then exp(-T_i) and a
+// zero pad (80 bytes, 16-byte aligned rows).
+#define QBX_BOYSL_ROWS 602
+#define QBX_BOYSL_STRIDE 10
+#define QBX_BOYSL_LMAX 8
 
 #define QBX_MAXL 2           // largest shell angular momentum in the kernels
 #define QBX_MAXC 6           // Cartesian components of the largest shell

@@ -1,12 +1,20 @@
 // qbx_eri.h -- two-electron kernels of libqbx_hip.so (gfx950).
 //
-// One thread evaluates one contracted shell quartet of class (LA LB|LC LD):
-// the primitive loop runs the generated vertical recurrence (EriClass::vrr)
-// and sums [e0|f0] over primitive quartets; the generated horizontal
-// recurrence (EriClass::hrr) then yields (ab|cd).  The epilogue depends on
-// MODE: Fock digest (J/K with FP64 atomics), dense tensor, or Schwarz bound.
-// Each class is instantiated in its own generated translation unit
-// (gen/eri_<class>.hip) so that the classes compile in parallel.
+// Work mapping (Fock digest and dense modes): one wavefront takes one bra
+// shell pair and up to 64 of its screened kets, one ket per lane.  The bra is
+// therefore wave-uniform: its primitive pairs, shells and density elements
+// come through scalar loads (constant address space) and sit in SGPRs, and
+// the J_ab contribution is summed across the wave before a single atomic.
+// Per lane, the primitive loop runs over the ket's primitive pairs (outer,
+// loaded once, next one prefetched) and the bra's (inner, scalar); each
+// primitive quartet evaluates the Boys function from the class's table rows
+// staged in LDS and runs the generated vertical recurrence (EriClass::vrr),
+// summing [e0|f0] over primitive quartets; the generated horizontal
+// recurrence then yields (ab|cd).  The epilogue depends on MODE: Fock digest
+// (J/K with FP64 atomics), dense tensor, or Schwarz bound (one thread per
+// diagonal pair, its own kernel).  Each class is instantiated in its own
+// generated translation unit (gen/eri_<class>.hip) so that the classes
+// compile in parallel.
 #pragma once
 #include "qbx.h"
 #include "qbx_common.h"
@@ -18,119 +26,218 @@
 template <int LA, int LB, int LC, int LD, typename R> struct EriClass;
 
 namespace qbx {
+
+// Loads through the constant address space become scalar (SMEM) loads when the
+// address is wave-uniform.  Only read-only data of the launch goes through it.
+template <typename T> using cptr = const __attribute__((address_space(4))) T*;
+template <typename T> __device__ __forceinline__ cptr<T> cst(const T* p) { return (cptr<T>)p; }
+
+// Threads per block: classes with few accumulators (and so few registers) run
+// 8 or 16 waves per block, so that one LDS copy of the Boys rows (48 KB, three
+// copies fit a CU) serves more waves.
+__host__ __device__ constexpr int eri_tpb(int nacc) { return nacc <= 3 ? 1024 : nacc <= 18 ? 512 : 256; }
+
+// Boys F_0..F_L from the LDS rows of order L (see qbx_device.h for the table):
+// an 8-term Taylor expansion of F_L about the nearest grid point, e^{-T} from
+// e^{-T_i} times the Taylor series of e^{T_i - T} (|T_i - T| <= 0.05, terms to
+// 8th order), the downward recurrence for the lower orders, and the asymptotic
+// form above the table.
+template <int L>
+__device__ __forceinline__ void boys_lds(double T, double* __restrict__ F, const double* __restrict__ sb)
+{
+    if (T < QBX_BOYS_TMAX) {
+        const int i = __double2int_rn(T * QBX_BOYS_INVDT);
+        const double md = (double)i * QBX_BOYS_DT - T;
+        const double2* row = (const double2*)(sb + i * QBX_BOYSL_STRIDE);
+        const double2 c01 = row[0], c23 = row[1], c45 = row[2], c67 = row[3];
+        double acc = fma(md, c67.y, c67.x);
+        acc = fma(md, acc, c45.y);
+        acc = fma(md, acc, c45.x);
+        acc = fma(md, acc, c23.y);
+        acc = fma(md, acc, c23.x);
+        acc = fma(md, acc, c01.y);
+        acc = fma(md, acc, c01.x);
+        F[L] = acc;
+        if (L > 0) {
+            const double ei = sb[i * QBX_BOYSL_STRIDE + QBX_BOYS_TAYLOR];
+            double ex = fma(md, 1.0 / 40320.0, 1.0 / 5040.0);
+            ex = fma(md, ex, 1.0 / 720.0);
+            ex = fma(md, ex, 1.0 / 120.0);
+            ex = fma(md, ex, 1.0 / 24.0);
+            ex = fma(md, ex, 1.0 / 6.0);
+            ex = fma(md, ex, 0.5);
+            ex = fma(md, ex, 1.0);
+            ex = fma(md, ex, 1.0);
+            const double e = ei * ex;
+            const double t2 = 2.0 * T;
+#pragma unroll
+            for (int m = L - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
+        }
+    } else {
+        const double rt = rsqrt(T);
+        F[0] = 0.88622692545275801365 * rt;
+        const double h = 0.5 * rt * rt;
+#pragma unroll
+        for (int m = 0; m < L; ++m) F[m + 1] = F[m] * ((2 * m + 1) * h);
+    }
+}
+
+// Copy the Boys rows of the class's total angular momentum into LDS.
+template <int TPB>
+__device__ __forceinline__ void stage_boys(double2* __restrict__ dst, const double* __restrict__ src)
+{
+    const double2* s = (const double2*)src;
+    for (int i = threadIdx.x; i < QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2; i += TPB) dst[i] = s[i];
+}
+
+// The ket-pair quantities one primitive quartet reads.
+struct KetPrim {
+    double q, inv2q, Qx, Qy, Qz, QCx, QCy, QCz, Kq;
+};
+
+__device__ __forceinline__ KetPrim load_ket(const PrimPair* __restrict__ p)
+{
+    KetPrim k;
+    k.q = p->p;
+    k.inv2q = p->inv2p;
+    k.Qx = p->Px;
+    k.Qy = p->Py;
+    k.Qz = p->Pz;
+    k.QCx = p->PAx;
+    k.QCy = p->PAy;
+    k.QCz = p->PAz;
+    k.Kq = p->Kp;
+    return k;
+}
+
+// Sum over the primitive quartets of one contracted shell quartet: bra pairs
+// BP[0..bn), ket pairs KP[0..kn).  BraPtr is a constant-address-space pointer
+// when the bra is wave-uniform (scalar loads), a plain pointer otherwise.
+template <class Cls, int LT, typename R, bool PREFETCH, typename BraPtr>
+__device__ __forceinline__ void prim_loop(R* __restrict__ acc, BraPtr BP, int bn,
+                                          const PrimPair* __restrict__ KP, int kn,
+                                          const double* __restrict__ sb)
+{
+    KetPrim nxt;
+    if (PREFETCH && kn > 0) nxt = load_ket(KP);
+    for (int v = 0; v < kn; ++v) {
+        KetPrim Q;
+        if (PREFETCH) {
+            Q = nxt;
+            if (v + 1 < kn) nxt = load_ket(KP + v + 1);
+        } else {
+            Q = load_ket(KP + v);
+        }
+        for (int u = 0; u < bn; ++u) {
+            const double p = BP[u].p, inv2p = BP[u].inv2p, Px = BP[u].Px, Py = BP[u].Py, Pz = BP[u].Pz;
+            const double PAx = BP[u].PAx, PAy = BP[u].PAy, PAz = BP[u].PAz, Kp = BP[u].Kp;
+            const double pq = p + Q.q;
+            const double s = rsqrt(pq);
+            const double inv = s * s;
+            const double PQx = Px - Q.Qx, PQy = Py - Q.Qy, PQz = Pz - Q.Qz;
+            const double rho = p * Q.q * inv;
+            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+            // 2 pi^{5/2} (c c K / p)_{ab} (c c K / q)_{cd} / sqrt(p+q)
+            const double pref = 34.986836655249725693 * Kp * Q.Kq * s;
+            double Fd[LT + 1];
+            boys_lds<LT>(T, Fd, sb);
+            R F[LT + 1];
+#pragma unroll
+            for (int m = 0; m <= LT; ++m) F[m] = (R)(Fd[m] * pref);
+            const double rho_p = Q.q * inv, rho_q = p * inv;
+            Cls::vrr(F, acc, (R)PAx, (R)PAy, (R)PAz,
+                     (R)(-rho_p * PQx), (R)(-rho_p * PQy), (R)(-rho_p * PQz),
+                     (R)Q.QCx, (R)Q.QCy, (R)Q.QCz,
+                     (R)(rho_q * PQx), (R)(rho_q * PQy), (R)(rho_q * PQz),
+                     (R)inv2p, (R)Q.inv2q, (R)(0.5 * inv), (R)rho_p, (R)rho_q);
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// J/K digest and dense tensor: one wave per (bra, chunk of 64 kets).
 template <int LA, int LB, int LC, int LD, typename R, int MODE>
-__global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
+__global__ __launch_bounds__(eri_tpb(EriClass<LA, LB, LC, LD, R>::NACC)) void eri_kernel(EriArgs a)
 {
     using Cls = EriClass<LA, LB, LC, LD, R>;
+    constexpr int TPB = eri_tpb(Cls::NACC);
     constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB), NC = qbx_ncart(LC), ND = qbx_ncart(LD);
     constexpr int LT = LA + LB + LC + LD;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.total) return;
-    int bi, ki;
-    if (MODE == MODE_SCHWARZ) {
-        bi = (int)t;
-        ki = (int)t;
-    } else {
-        int lo = 0, hi = a.nbra;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (a.prefix[mid] <= t) lo = mid; else hi = mid;
-        }
-        bi = lo;
-        ki = a.koff[lo] + (int)(t - a.prefix[lo]);
-    }
+    constexpr int NCD = NC * ND;
+    constexpr bool PREFETCH = Cls::NACC <= 18;
+    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
+    stage_boys<TPB>(sboys, a.boysl);
+    __syncthreads();
+    const double* sb = (const double*)sboys;
 
+    const int lane = (int)threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)blockIdx.x * (TPB / 64) + ((int)threadIdx.x >> 6));
+    if (wave >= a.nwave) return;
+    // bra: the last one whose first wave is <= this wave (bras without kets own no wave)
+    const cptr<int> cpre = cst(a.cpre);
+    int lo = 0, hi = a.nbra;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cpre[mid] <= wave) lo = mid; else hi = mid;
+    }
+    const int bi = lo;
+    const int nk = (int)(cst(a.prefix)[bi + 1] - cst(a.prefix)[bi]);
+    const int pos = (wave - cpre[bi]) * 64 + lane;
+    const bool inrange = pos < nk;
+    const int ki = cst(a.koff)[bi] + (inrange ? pos : 0);
+
+    const int sA = cst(a.bra.shA)[bi], sB = cst(a.bra.shB)[bi];
+    const int sC = a.ket.shA[ki], sD = a.ket.shB[ki];
+    int kn = inrange ? a.ket.ppn[ki] : 0;
     if (MODE == MODE_JK && a.pdmax) {
         // Haser-Ahlrichs bound with the densities of this build
-        const int xA = a.bra.shA[bi], xB = a.bra.shB[bi], xC = a.ket.shA[ki], xD = a.ket.shB[ki];
         const double* dm = a.pdmax;
         const int ns = a.nshell;
-        double d = fmax(fmax(dm[xA * ns + xB], dm[xC * ns + xD]), fmax(dm[xA * ns + xC], dm[xA * ns + xD]));
-        d = fmax(d, fmax(dm[xB * ns + xC], dm[xB * ns + xD]));
-        if (a.bra.Q[bi] * a.ket.Q[ki] * d < a.dthr) return;
+        const double dab = cst(dm)[sA * ns + sB];
+        double d = fmax(fmax(dab, dm[sC * ns + sD]), fmax(dm[sA * ns + sC], dm[sA * ns + sD]));
+        d = fmax(d, fmax(dm[sB * ns + sC], dm[sB * ns + sD]));
+        if (cst(a.bra.Q)[bi] * a.ket.Q[ki] * d < a.dthr) kn = 0;
     }
+    if (__ballot(kn > 0) == 0) return;
+    const bool live = kn > 0;
 
     R acc[Cls::NACC];
 #pragma unroll
     for (int n = 0; n < Cls::NACC; ++n) acc[n] = (R)0;
+    prim_loop<Cls, LT, R, PREFETCH>(acc, cst(a.pp + cst(a.bra.ppo)[bi]), cst(a.bra.ppn)[bi],
+                                    a.pp + a.ket.ppo[ki], kn, sb);
 
-    const int bo = a.bra.ppo[bi], bn = a.bra.ppn[bi];
-    const int ko = a.ket.ppo[ki], kn = a.ket.ppn[ki];
-    for (int u = 0; u < bn; ++u) {
-        const PrimPair P = a.pp[bo + u];
-        for (int v = 0; v < kn; ++v) {
-            const PrimPair Q = a.pp[ko + v];
-            const double pq = P.p + Q.p;
-            const double s = rsqrt(pq);
-            const double inv = s * s;
-            const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
-            const double rho = P.p * Q.p * inv;
-            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            // 2 pi^{5/2} (c c K / p)_{ab} (c c K / q)_{cd} / sqrt(p+q)
-            const double pref = 34.986836655249725693 * P.Kp * Q.Kp * s;
-            double Fd[LT + 1];
-            qbx_boys<LT>(T, Fd, a.boys, a.bexp);
-            R F[LT + 1];
-#pragma unroll
-            for (int m = 0; m <= LT; ++m) F[m] = (R)(Fd[m] * pref);
-            const double rho_p = Q.p * inv, rho_q = P.p * inv;
-            Cls::vrr(F, acc, (R)P.PAx, (R)P.PAy, (R)P.PAz,
-                     (R)(-rho_p * PQx), (R)(-rho_p * PQy), (R)(-rho_p * PQz),
-                     (R)Q.PAx, (R)Q.PAy, (R)Q.PAz,
-                     (R)(rho_q * PQx), (R)(rho_q * PQy), (R)(rho_q * PQz),
-                     (R)P.inv2p, (R)Q.inv2p, (R)(0.5 * inv), (R)rho_p, (R)rho_q);
-        }
-    }
-
-    constexpr int NCD = NC * ND;
-    const double* ABp = a.bra.AB + 3 * bi;
+    const cptr<double> ABp = cst(a.bra.AB + 3 * bi);
     const double* CDp = a.ket.AB + 3 * ki;
     R kacc[Cls::NE * NCD];
     Cls::hrr_ket(acc, kacc, (R)CDp[0], (R)CDp[1], (R)CDp[2]);
     const R ABx = (R)ABp[0], ABy = (R)ABp[1], ABz = (R)ABp[2];
 
-    const int sA = a.bra.shA[bi], sB = a.bra.shB[bi];
-    const int sC = a.ket.shA[ki], sD = a.ket.shB[ki];
-    const DevShell& SA = a.sh[sA];
-    const DevShell& SB = a.sh[sB];
-    const DevShell& SC = a.sh[sC];
-    const DevShell& SD = a.sh[sD];
-    int fa[NA], fb[NB], fc[NC], fd[ND];
+    const cptr<DevShell> SA = cst(a.sh + sA);
+    const cptr<DevShell> SB = cst(a.sh + sB);
+    const DevShell* SC = a.sh + sC;
+    const DevShell* SD = a.sh + sD;
+    int fc[NC], fd[ND];
 #pragma unroll
-    for (int n = 0; n < NA; ++n) fa[n] = SA.bf[n];
+    for (int n = 0; n < NC; ++n) fc[n] = SC->bf[n];
 #pragma unroll
-    for (int n = 0; n < NB; ++n) fb[n] = SB.bf[n];
-#pragma unroll
-    for (int n = 0; n < NC; ++n) fc[n] = SC.bf[n];
-#pragma unroll
-    for (int n = 0; n < ND; ++n) fd[n] = SD.bf[n];
+    for (int n = 0; n < ND; ++n) fd[n] = SD->bf[n];
     // product of the ket component scales (weight ratios inside a shell)
     double scd[NCD];
 #pragma unroll
     for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
-        for (int id = 0; id < ND; ++id) scd[ic * ND + id] = SC.sc[ic] * SD.sc[id];
-
-    if (MODE == MODE_SCHWARZ) {
-        // bra == ket: the diagonal (ab|ab) elements of each slice a
-        double mx = 0.0;
-#pragma unroll
-        for (int ia = 0; ia < NA; ++ia) {
-            R out[NB * NCD];
-            Cls::hrr_bra(ia, kacc, out, ABx, ABy, ABz);
-#pragma unroll
-            for (int ib = 0; ib < NB; ++ib) {
-                const double w = SA.sc[ia] * SB.sc[ib];
-                const double v = fabs((double)out[ib * NCD + ia * ND + ib] * w * w);
-                mx = v > mx ? v : mx;
-            }
-        }
-        a.qout[bi] = sqrt(mx);
-        return;
-    }
+        for (int id = 0; id < ND; ++id) scd[ic * ND + id] = SC->sc[ic] * SD->sc[id];
 
     const size_t N = (size_t)a.N;
     if (MODE == MODE_DENSE) {
+        if (!live) return;
 #pragma unroll
         for (int ia = 0; ia < NA; ++ia) {
             R out[NB * NCD];
@@ -141,9 +248,9 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
                 for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
                     for (int id = 0; id < ND; ++id) {
-                        if (fa[ia] < 0 || fb[ib] < 0 || fc[ic] < 0 || fd[id] < 0) continue;
-                        const size_t i = fa[ia], j = fb[ib], k = fc[ic], l = fd[id];
-                        const double v = (double)out[ib * NCD + ic * ND + id] * (SA.sc[ia] * SB.sc[ib]) *
+                        if (SA->bf[ia] < 0 || SB->bf[ib] < 0 || fc[ic] < 0 || fd[id] < 0) continue;
+                        const size_t i = SA->bf[ia], j = SB->bf[ib], k = fc[ic], l = fd[id];
+                        const double v = (double)out[ib * NCD + ic * ND + id] * (SA->sc[ia] * SB->sc[ib]) *
                                          scd[ic * ND + id];
                         a.eri[((i * N + j) * N + k) * N + l] = v;
                         a.eri[((j * N + i) * N + k) * N + l] = v;
@@ -160,16 +267,17 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
 
     // ---- MODE_JK: digest, one bra component a at a time, with the
     //      degeneracy of the unique shell quartet ----
-    double f = 1.0;
+    double f = live ? 1.0 : 0.0;
     if (sA == sB) f *= 0.5;
     if (sC == sD) f *= 0.5;
     if (a.same && bi == ki) f *= 0.5;
     const double fj = 4.0 * f, fk = 2.0 * f;
     // missing components carry scale 0 and point at function 0
+    int fa[NA], fb[NB];
 #pragma unroll
-    for (int n = 0; n < NA; ++n) fa[n] = fa[n] < 0 ? 0 : fa[n];
+    for (int n = 0; n < NA; ++n) fa[n] = SA->bf[n] < 0 ? 0 : SA->bf[n];
 #pragma unroll
-    for (int n = 0; n < NB; ++n) fb[n] = fb[n] < 0 ? 0 : fb[n];
+    for (int n = 0; n < NB; ++n) fb[n] = SB->bf[n] < 0 ? 0 : SB->bf[n];
 #pragma unroll
     for (int n = 0; n < NC; ++n) fc[n] = fc[n] < 0 ? 0 : fc[n];
 #pragma unroll
@@ -213,8 +321,8 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
             for (int id = 0; id < ND; ++id) { dad[id] = DK[fa[ia] * N + fd[id]]; kad[id] = 0.0; }
 #pragma unroll
             for (int ib = 0; ib < NB; ++ib) {
-                const double sab = SA.sc[ia] * SB.sc[ib];
-                const double dab = DJ[fa[ia] * N + fb[ib]];
+                const double sab = SA->sc[ia] * SB->sc[ib];
+                const double dab = cst(DJ)[fa[ia] * N + fb[ib]];
                 double jab = 0.0;
 #pragma unroll
                 for (int ic = 0; ic < NC; ++ic)
@@ -229,45 +337,100 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
                         kbd[ib * ND + id] += v * dac[ic];
                         kbc[ib * NC + ic] += v * dad[id];
                     }
-                atomicAdd(Js + fa[ia] * N + fb[ib], fj * jab);
+                // the bra is wave-uniform: one atomic per wave for J_ab
+                jab = wave_sum(fj * jab);
+                if (lane == 0) atomicAdd(Js + fa[ia] * N + fb[ib], jab);
             }
+            if (live) {
 #pragma unroll
-            for (int id = 0; id < ND; ++id) atomicAdd(Ks + fa[ia] * N + fd[id], fk * kad[id]);
+                for (int id = 0; id < ND; ++id) atomicAdd(Ks + fa[ia] * N + fd[id], fk * kad[id]);
 #pragma unroll
-            for (int ic = 0; ic < NC; ++ic) atomicAdd(Ks + fa[ia] * N + fc[ic], fk * kac[ic]);
+                for (int ic = 0; ic < NC; ++ic) atomicAdd(Ks + fa[ia] * N + fc[ic], fk * kac[ic]);
+            }
         }
+        if (live) {
 #pragma unroll
-        for (int ic = 0; ic < NC; ++ic)
+            for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
-            for (int id = 0; id < ND; ++id) atomicAdd(Js + fc[ic] * N + fd[id], fj * jcd[ic * ND + id]);
+                for (int id = 0; id < ND; ++id) atomicAdd(Js + fc[ic] * N + fd[id], fj * jcd[ic * ND + id]);
 #pragma unroll
-        for (int ib = 0; ib < NB; ++ib)
+            for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
-            for (int id = 0; id < ND; ++id) atomicAdd(Ks + fb[ib] * N + fd[id], fk * kbd[ib * ND + id]);
+                for (int id = 0; id < ND; ++id) atomicAdd(Ks + fb[ib] * N + fd[id], fk * kbd[ib * ND + id]);
 #pragma unroll
-        for (int ib = 0; ib < NB; ++ib)
+            for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
-            for (int ic = 0; ic < NC; ++ic) atomicAdd(Ks + fb[ib] * N + fc[ic], fk * kbc[ib * NC + ic]);
+                for (int ic = 0; ic < NC; ++ic) atomicAdd(Ks + fb[ib] * N + fc[ic], fk * kbc[ib * NC + ic]);
+        }
     }
+}
+
+// Schwarz bounds: one thread per diagonal shell pair (ab|ab).
+template <int LA, int LB, typename R>
+__global__ __launch_bounds__(256) void eri_schwarz_kernel(EriArgs a)
+{
+    using Cls = EriClass<LA, LB, LA, LB, R>;
+    constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB);
+    constexpr int LT = 2 * (LA + LB);
+    constexpr int NCD = NA * NB;
+    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
+    stage_boys<256>(sboys, a.boysl);
+    __syncthreads();
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.total) return;
+    const int bi = (int)t;
+    R acc[Cls::NACC];
+#pragma unroll
+    for (int n = 0; n < Cls::NACC; ++n) acc[n] = (R)0;
+    const PrimPair* P = a.pp + a.bra.ppo[bi];
+    const int n = a.bra.ppn[bi];
+    prim_loop<Cls, LT, R, false>(acc, P, n, P, n, (const double*)sboys);
+    const double* AB = a.bra.AB + 3 * bi;
+    R kacc[Cls::NE * NCD];
+    Cls::hrr_ket(acc, kacc, (R)AB[0], (R)AB[1], (R)AB[2]);
+    const DevShell& SA = a.sh[a.bra.shA[bi]];
+    const DevShell& SB = a.sh[a.bra.shB[bi]];
+    double mx = 0.0;
+#pragma unroll
+    for (int ia = 0; ia < NA; ++ia) {
+        R out[NB * NCD];
+        Cls::hrr_bra(ia, kacc, out, (R)AB[0], (R)AB[1], (R)AB[2]);
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib) {
+            const double w = SA.sc[ia] * SB.sc[ib];
+            const double v = fabs((double)out[ib * NCD + ia * NB + ib] * w * w);
+            mx = v > mx ? v : mx;
+        }
+    }
+    a.qout[bi] = sqrt(mx);
 }
 
 
 template <int LA, int LB, int LC, int LD, typename R, int MODE>
 void launch_eri(const EriArgs& a, hipStream_t s)
 {
-    if (a.total <= 0) return;
-    // Kernels that spill to scratch run on the caller's main stream only: each
-    // extra hardware queue would need its own scratch allocation.
-    static const size_t scratch = [] {
-        hipFuncAttributes fa;
-        if (hipFuncGetAttributes(&fa, (const void*)eri_kernel<LA, LB, LC, LD, R, MODE>) != hipSuccess) return (size_t)1;
-        return (size_t)fa.localSizeBytes;
-    }();
-    hipStream_t use = (scratch > 0 && a.scratch_stream) ? (hipStream_t)a.scratch_stream : s;
-    const int threads = 256;
-    const int64_t blocks = (a.total + threads - 1) / threads;
-    hipLaunchKernelGGL((eri_kernel<LA, LB, LC, LD, R, MODE>), dim3((unsigned)blocks), dim3(threads), 0, use, a);
-    QBX_HIP(hipGetLastError());
+    if constexpr (MODE == MODE_SCHWARZ) {
+        static_assert(LA == LC && LB == LD, "Schwarz bounds need a diagonal class");
+        if (a.total <= 0) return;
+        const int64_t blocks = (a.total + 255) / 256;
+        hipLaunchKernelGGL((eri_schwarz_kernel<LA, LB, R>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+        QBX_HIP(hipGetLastError());
+    } else {
+        if (a.nwave <= 0) return;
+        // Kernels that spill to scratch run on the caller's main stream only:
+        // each extra hardware queue would need its own scratch allocation.
+        static const size_t scratch = [] {
+            hipFuncAttributes fa;
+            if (hipFuncGetAttributes(&fa, (const void*)eri_kernel<LA, LB, LC, LD, R, MODE>) != hipSuccess)
+                return (size_t)1;
+            return (size_t)fa.localSizeBytes;
+        }();
+        hipStream_t use = (scratch > 0 && a.scratch_stream) ? (hipStream_t)a.scratch_stream : s;
+        constexpr int TPB = eri_tpb(EriClass<LA, LB, LC, LD, R>::NACC);
+        const int64_t blocks = ((int64_t)a.nwave + TPB / 64 - 1) / (TPB / 64);
+        hipLaunchKernelGGL((eri_kernel<LA, LB, LC, LD, R, MODE>), dim3((unsigned)blocks), dim3(TPB), 0, use, a);
+        QBX_HIP(hipGetLastError());
+    }
 }
 
 }  // namespace qbx

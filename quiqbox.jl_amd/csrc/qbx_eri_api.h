@@ -23,9 +23,12 @@ struct EriArgs {
     const DevShell* sh;
     const double* boys;
     const double* bexp;
-    const int64_t* prefix;
-    const int* koff;
-    int64_t total;
+    const double* boysl;      // per-order Boys rows of the class's total L
+    const int64_t* prefix;    // [nbra+1] prefix over bras of the screened ket counts
+    const int* koff;          // [nbra] first ket of each bra
+    const int* cpre;          // [nbra+1] prefix over bras of ceil(kets / 64): waves
+    int64_t total;            // Schwarz mode: number of diagonal pairs
+    int nwave;                // J/K and dense modes: number of waves
     int nbra;
     int same;
     const double* D;   // [2*nset][N][N]: DJ sets then DK sets
