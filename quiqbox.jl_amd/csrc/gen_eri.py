@@ -28,6 +28,14 @@ Obara-Saika recurrences, J. Chem. Phys. 84, 3963 (1986)):
 Cartesian components follow the order used by Quiqbox (Lexicons.jl:17-36):
 for i = L..0, for j = L-i..0: (i, j, L-i-j).
 
+Staged classes (the largest: their [e0|f0] accumulators do not fit the
+register file) are emitted in groups of bra components e: vrr_grp<g> sums the
+targets of group g only, hrr_ket_grp<g> transfers that group to the ket, and
+the caller stores each group's [e|cd) to a per-wave workspace before the next
+group's pass over the primitive quartets.  hrr_bra_ab then forms one
+(a b | c d) row for all cd from the workspace, the bra transfer written out
+as polynomials in (A-B):  (a,b| = sum_e P_e(A-B) (e,0|.
+
 The generator also writes eri_flops.json: arithmetic operations per primitive
 quartet (VRR + Boys seed) and per contracted quartet (HRR) for each class,
 counted from the emitted expressions (add/sub/mul = 1, fma pattern a*b+c
@@ -39,6 +47,9 @@ import os
 import sys
 
 LMAX = 2
+# classes evaluated in groups of bra components through a workspace
+STAGED = {(2, 1, 1, 1), (2, 1, 2, 1), (2, 2, 1, 1), (2, 2, 2, 0), (2, 2, 2, 1), (2, 2, 2, 2)}
+STAGE_ACC = 32     # accumulators per group
 
 
 def cart(L):
@@ -142,6 +153,159 @@ def gen_vrr(em, targets):
     return out
 
 
+def bra_expansion(a, b):
+    """(a b| as {e: {(px, py, pz): integer coefficient}}: the bra transfer
+    (a, b+1_i| = (a+1_i, b| + (A-B)_i (a, b| unrolled down to b = 0."""
+    if sum(b) == 0:
+        return {a: {(0, 0, 0): 1}}
+    i = pick_axis(b)
+    b1 = sub1(b, i)
+    res = {e: dict(p) for e, p in bra_expansion(add1(a, i), b1).items()}
+    for e, poly in bra_expansion(a, b1).items():
+        tgt = res.setdefault(e, {})
+        for mono, c in poly.items():
+            m2 = add1(mono, i)
+            tgt[m2] = tgt.get(m2, 0) + c
+    return res
+
+
+def mono_expr(mono):
+    f = []
+    for ax, pw in zip(AX, mono):
+        f += ["AB%s" % ax] * pw
+    return " * ".join(f) if f else "(R)1"
+
+
+def gen_staged(la, lb, lc, ld, es, fs, eidx, fidx, C_, D_):
+    """Grouped VRR / ket HRR and the workspace bra HRR of a staged class."""
+    nE, nF = len(es), len(fs)
+    nC, nD = len(C_), len(D_)
+    NCD = nC * nD
+    G = -(-nE * nF // STAGE_ACC)
+    G = min(G, nE)
+    bounds = [k * nE // G for k in range(G + 1)]
+    code = []
+    flops_vrr_exec = 0
+    code.append("    static constexpr bool STAGED = true;")
+    code.append("    static constexpr int NGRP = %d;" % G)
+    code.append("    static constexpr int NCD = %d;" % NCD)
+    code.append("    // group g holds bra components e in [ge0(g), ge0(g + 1))")
+    sel = " : ".join("g == %d ? %d" % (k, bounds[k]) for k in range(G)) + " : %d" % nE
+    code.append("    static constexpr int ge0(int g) { return %s; }" % sel)
+    code.append("    static constexpr int GMAXACC = %d;" % (max(bounds[k + 1] - bounds[k] for k in range(G)) * nF))
+    for g in range(G):
+        eg = es[bounds[g]:bounds[g + 1]]
+        em = Emitter()
+        targets = [(e, f) for e in eg for f in fs]
+        vv = gen_vrr(em, targets)
+        lines = list(em.lines)
+        for (e, f) in targets:
+            lines.append("acc[%d] += %s;" % ((eidx[e] - bounds[g]) * nF + fidx[f], vv[(e, f)]))
+        flops_vrr_exec += em.flops + len(targets)
+        code.append("    static __device__ __forceinline__ void vrr_g%d(const R* __restrict__ F, R* __restrict__ acc," % g)
+        code.append("        R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz,")
+        code.append("        R WQx, R WQy, R WQz, R inv2p, R inv2q, R inv2pq, R rho_p, R rho_q) {")
+        code += ["        " + l for l in lines]
+        code.append("    }")
+        # ket transfer of this group: kout[(e - ge0) * NCD + cd]
+        hm = Emitter()
+        memo = {}
+
+        def K(e, c, d):
+            key = (e, c, d)
+            if key in memo:
+                return memo[key]
+            if sum(d) == 0:
+                r = "acc[%d]" % ((e - bounds[g]) * nF + fidx[c])
+                memo[key] = r
+                return r
+            i = pick_axis(d)
+            d1 = sub1(d, i)
+            x = K(e, add1(c, i), d1)
+            y = K(e, c, d1)
+            name = hm.fresh("k")
+            hm.lines.append("const R %s = %s + CD%s * %s;" % (name, x, AX[i], y))
+            hm.flops += 2
+            memo[key] = name
+            return name
+
+        outs = []
+        for ie in range(bounds[g], bounds[g + 1]):
+            for ic, c in enumerate(C_):
+                for idd, d in enumerate(D_):
+                    outs.append("ws.st(%d, %s);" % (ie * NCD + ic * nD + idd, K(ie, c, d)))
+        code.append("    template <class W> static __device__ __forceinline__ void hrr_ket_g%d(const R* __restrict__ acc, "
+                    "const W& ws, R CDx, R CDy, R CDz) {" % g)
+        code += ["        " + l for l in hm.lines + outs]
+        code.append("    }")
+    args = "F, acc, PAx, PAy, PAz, WPx, WPy, WPz, QCx, QCy, QCz, WQx, WQy, WQz, inv2p, inv2q, inv2pq, rho_p, rho_q"
+    code.append("    template <int G> static __device__ __forceinline__ void vrr_grp(const R* __restrict__ F, "
+                "R* __restrict__ acc,")
+    code.append("        R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz,")
+    code.append("        R WQx, R WQy, R WQz, R inv2p, R inv2q, R inv2pq, R rho_p, R rho_q) {")
+    for g in range(G):
+        code.append("        %sif constexpr (G == %d) vrr_g%d(%s);" % ("" if g == 0 else "else ", g, g, args))
+    code.append("    }")
+    code.append("    // workspace value e * NCD + cd = [e | c d) for the bra components e of group G")
+    code.append("    template <int G, class W> static __device__ __forceinline__ void hrr_ket_grp(const R* __restrict__ acc, "
+                "const W& ws, R CDx, R CDy, R CDz) {")
+    for g in range(G):
+        code.append("        %sif constexpr (G == %d) hrr_ket_g%d(acc, ws, CDx, CDy, CDz);" % (
+            "" if g == 0 else "else ", g, g))
+    code.append("    }")
+    # bra transfer rows from the workspace: ws[(e * NCD + cd) * WS_STRIDE]
+    A_, B_ = cart(la), cart(lb)
+    nB = len(B_)
+    cases = []
+    flops_bra = 0
+    for ia, a0 in enumerate(A_):
+        for ib, b0 in enumerate(B_):
+            ex = bra_expansion(a0, b0)
+            terms = []
+            pre = []
+            for e in sorted(ex, key=lambda t: eidx[t]):
+                for mono, c in sorted(ex[e].items()):
+                    if c == 0:
+                        continue
+                    terms.append((eidx[e], mono, c))
+            body = []
+            names = {}
+            for (ei, mono, c) in terms:
+                if (mono, c) not in names and not (sum(mono) == 0 and c == 1):
+                    nm = "c%d" % len(names)
+                    names[(mono, c)] = nm
+                    expr = mono_expr(mono)
+                    pre.append("const R %s = (R)%d * %s;" % (nm, c, expr) if c != 1 else "const R %s = %s;" % (nm, expr))
+            sum_terms = []
+            for (ei, mono, c) in terms:
+                w = "ws.ld(%d * NCD + cd)" % ei
+                sum_terms.append(w if (sum(mono) == 0 and c == 1) else "%s * %s" % (names[(mono, c)], w))
+            flops_bra += NCD * (2 * len(terms) - 1)
+            body += pre
+            body.append("#pragma unroll")
+            body.append("for (int cd = 0; cd < NCD; ++cd) out[cd] = %s;" % " + ".join(sum_terms))
+            cases.append("        case %d: {\n%s\n            break;\n        }" % (
+                ia * nB + ib, "\n".join("            " + l for l in body)))
+    code.append("    // out[cd] = (a b | c d) for bra components (ia, ib), iab = ia * NB + ib, from the")
+    code.append("    // workspace values e * NCD + cd = [e | c d)")
+    code.append("    template <class W> static __device__ __forceinline__ void hrr_bra_ab(int iab, "
+                "const W& ws, R* __restrict__ out, R ABx, R ABy, R ABz) {")
+    code.append("        switch (iab) {")
+    code += cases
+    code.append("        default: break;")
+    code.append("        }")
+    code.append("    }")
+    return code, flops_vrr_exec, flops_bra
+
+
+def stage_ws(la, lb, lc, ld):
+    """[e|cd) values per lane that a staged class keeps in its workspace."""
+    if (la, lb, lc, ld) not in STAGED:
+        return 0
+    nE = sum(ncart(L) for L in range(la, la + lb + 1))
+    return nE * ncart(lc) * ncart(ld)
+
+
 def class_list():
     pcl = [(a, b) for a in range(LMAX + 1) for b in range(a + 1)]
     order = {c: n for n, c in enumerate(pcl)}
@@ -210,6 +374,7 @@ def gen_class(la, lb, lc, ld):
     A_, B_ = cart(la), cart(lb)
     nB = len(B_)
     cases = []
+    sum_bra_flops = [0]
     for ia, a0 in enumerate(A_):
         bm = Emitter()
         memo = {}
@@ -237,6 +402,7 @@ def gen_class(la, lb, lc, ld):
             for cd in range(NCD):
                 outs.append("out[%d] = %s;" % (ib * NCD + cd, Bv(a0, b0, cd)))
         flops_hrr += bm.flops
+        sum_bra_flops[0] += bm.flops
         body = "\n".join("            " + l for l in bm.lines + outs)
         cases.append("        case %d: {\n%s\n            break;\n        }" % (ia, body))
     bra_code = "        switch (ia) {\n" + "\n".join(cases) + "\n        default: break;\n        }"
@@ -247,6 +413,13 @@ def gen_class(la, lb, lc, ld):
     code.append("    static constexpr int NACC = %d;" % (nE * nF))
     code.append("    static constexpr int NE = %d;" % nE)
     code.append("    static constexpr int LT = %d;" % Lt)
+    flops_exec = None
+    if (la, lb, lc, ld) in STAGED:
+        scode, fve, fbra = gen_staged(la, lb, lc, ld, es, fs, eidx, fidx, C_, D_)
+        code += scode
+        flops_exec = (fve, flops_hrr - sum_bra_flops[0] + fbra)
+    else:
+        code.append("    static constexpr bool STAGED = false;")
     code.append("    static __device__ __forceinline__ void vrr(const R* __restrict__ F, R* __restrict__ acc,")
     code.append("        R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz,")
     code.append("        R WQx, R WQy, R WQz, R inv2p, R inv2q, R inv2pq, R rho_p, R rho_q) {")
@@ -263,7 +436,7 @@ def gen_class(la, lb, lc, ld):
     code.append(bra_code)
     code.append("    }")
     code.append("};")
-    return "\n".join(code), flops_vrr, flops_hrr, nE * nF
+    return "\n".join(code), flops_vrr, flops_hrr, nE * nF, flops_exec
 
 
 def main(outdir):
@@ -271,9 +444,16 @@ def main(outdir):
     classes, pcl = class_list()
     flops = {}
     for (la, lb, lc, ld) in classes:
-        code, fv, fh, nacc = gen_class(la, lb, lc, ld)
+        code, fv, fh, nacc, fexec = gen_class(la, lb, lc, ld)
         tag = "%d%d%d%d" % (la, lb, lc, ld)
-        flops[tag] = {"vrr_per_prim_quartet": fv, "hrr_per_quartet": fh, "n_acc": nacc}
+        flops[tag] = {"vrr_per_prim_quartet": fv, "hrr_per_quartet": fh, "n_acc": nacc,
+                      "staged": fexec is not None}
+        if fexec is not None:
+            # executed counts of the grouped evaluation (the VRR intermediates
+            # shared between groups are recomputed); the figures above stay the
+            # algorithmic counts of the ungrouped recurrence
+            flops[tag]["vrr_executed_per_prim_quartet"] = fexec[0]
+            flops[tag]["hrr_executed_per_quartet"] = fexec[1]
         variants = [("double", "MODE_JK", "jk64"), ("float", "MODE_JK", "jk32"), ("double", "MODE_DENSE", "dense")]
         if (la, lb) == (lc, ld):
             variants.append(("double", "MODE_SCHWARZ", "schwarz"))
@@ -296,6 +476,9 @@ def main(outdir):
                  len(pcl), ", ".join("{%d, %d}" % c for c in pcl)),
              "static const int qbx_class_l[%d][4] = {%s};" % (
                  len(classes), ", ".join("{%d, %d, %d, %d}" % c for c in classes)),
+             "// workspace values per lane of the staged classes (0: not staged)",
+             "static const int qbx_class_ws[%d] = {%s};" % (
+                 len(classes), ", ".join(str(stage_ws(*c)) for c in classes)),
              "#define QBX_FOR_EACH_CLASS(X) \\"]
     lines += ["    X(%d, %d, %d, %d) \\" % c for c in classes]
     lines.append("")

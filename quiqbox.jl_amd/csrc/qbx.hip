@@ -95,6 +95,8 @@ template <typename T> struct DevBuf {
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr int kStreams = 4;
+// wave slots of a staged class's workspace (persistent grid): 8 per CU
+constexpr int kStageSlots = 2048;
 // primitive pairs with |c_a c_b exp(-ab/p R^2) / p| below this are dropped
 constexpr double QBX_PRIM_PAIR_CUT = 1e-30;
 
@@ -163,6 +165,7 @@ struct qbx_basis {
     double task_thr = -1.0, task_cut = -1.0;
     int task_mode = -1;
     TaskSet t64[QBX_N_CLASSES], t32[QBX_N_CLASSES];
+    DevBuf<double> ws[QBX_N_CLASSES];   // staged-class workspaces
     // work buffers
     DevBuf<double> d_D, d_J, d_K, d_G;
     DevBuf<double> d_pdmax;        // shell-pair max |D| for density screening
@@ -456,6 +459,18 @@ EriArgs base_args(qbx_basis* b, int cls)
     return a;
 }
 
+// Workspace of a staged class for up to `nwave` waves (kStageSlots at most).
+void attach_workspace(qbx_basis* b, int cls, int nwave, EriArgs& a)
+{
+    const int wsv = qbx_class_ws[cls];
+    if (wsv == 0 || nwave <= 0) return;
+    int slots = std::min(nwave, kStageSlots);
+    slots = (slots + 3) / 4 * 4;
+    b->ws[cls].ensure((size_t)slots * wsv * 64);
+    a.ws = b->ws[cls].p;
+    a.ws_slots = (int)(b->ws[cls].n / ((size_t)wsv * 64));
+}
+
 // Schwarz bounds, then every pair class sorted by descending bound.
 void ensure_schwarz(qbx_basis* b)
 {
@@ -716,6 +731,7 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
         a.total = t64.total;
         a.nwave = t64.nwave;
         a.nbra = t64.nbra;
+        attach_workspace(b, cls, std::max(t64.nwave, mode == QBX_PREC_MIXED ? b->t32[cls].nwave : 0), a);
         k_jk64[cls](a, ls);
         if (mode == QBX_PREC_MIXED) {
             const TaskSet& t32 = b->t32[cls];
@@ -1092,6 +1108,7 @@ int qbx_eri_dense(qbx_ctx* ctx, const qbx_basis* bc, double* eri)
             a.total = tc.p64[nx];
             a.nwave = tc.w64[nx];
             a.nbra = nx;
+            attach_workspace(b, cls, a.nwave, a);
             a.eri = d.p;
             k_dense[cls](a, st);
             QBX_HIP(hipStreamSynchronize(st));

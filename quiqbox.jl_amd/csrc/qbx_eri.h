@@ -22,6 +22,7 @@
 #include "qbx_eri_api.h"
 
 #include <cstdint>
+#include <utility>
 
 template <int LA, int LB, int LC, int LD, typename R> struct EriClass;
 
@@ -113,7 +114,8 @@ __device__ __forceinline__ KetPrim load_ket(const PrimPair* __restrict__ p)
 // Sum over the primitive quartets of one contracted shell quartet: bra pairs
 // BP[0..bn), ket pairs KP[0..kn).  BraPtr is a constant-address-space pointer
 // when the bra is wave-uniform (scalar loads), a plain pointer otherwise.
-template <class Cls, int LT, typename R, bool PREFETCH, typename BraPtr>
+// GRP < 0 runs the full recurrence, GRP >= 0 that group of a staged class.
+template <class Cls, int LT, typename R, bool PREFETCH, int GRP, typename BraPtr>
 __device__ __forceinline__ void prim_loop(R* __restrict__ acc, BraPtr BP, int bn,
                                           const PrimPair* __restrict__ KP, int kn,
                                           const double* __restrict__ sb)
@@ -145,11 +147,18 @@ __device__ __forceinline__ void prim_loop(R* __restrict__ acc, BraPtr BP, int bn
 #pragma unroll
             for (int m = 0; m <= LT; ++m) F[m] = (R)(Fd[m] * pref);
             const double rho_p = Q.q * inv, rho_q = p * inv;
-            Cls::vrr(F, acc, (R)PAx, (R)PAy, (R)PAz,
-                     (R)(-rho_p * PQx), (R)(-rho_p * PQy), (R)(-rho_p * PQz),
-                     (R)Q.QCx, (R)Q.QCy, (R)Q.QCz,
-                     (R)(rho_q * PQx), (R)(rho_q * PQy), (R)(rho_q * PQz),
-                     (R)inv2p, (R)Q.inv2q, (R)(0.5 * inv), (R)rho_p, (R)rho_q);
+            if constexpr (GRP < 0)
+                Cls::vrr(F, acc, (R)PAx, (R)PAy, (R)PAz,
+                         (R)(-rho_p * PQx), (R)(-rho_p * PQy), (R)(-rho_p * PQz),
+                         (R)Q.QCx, (R)Q.QCy, (R)Q.QCz,
+                         (R)(rho_q * PQx), (R)(rho_q * PQy), (R)(rho_q * PQz),
+                         (R)inv2p, (R)Q.inv2q, (R)(0.5 * inv), (R)rho_p, (R)rho_q);
+            else
+                Cls::template vrr_grp<GRP>(F, acc, (R)PAx, (R)PAy, (R)PAz,
+                         (R)(-rho_p * PQx), (R)(-rho_p * PQy), (R)(-rho_p * PQz),
+                         (R)Q.QCx, (R)Q.QCy, (R)Q.QCz,
+                         (R)(rho_q * PQx), (R)(rho_q * PQy), (R)(rho_q * PQz),
+                         (R)inv2p, (R)Q.inv2q, (R)(0.5 * inv), (R)rho_p, (R)rho_q);
         }
     }
 }
@@ -210,7 +219,7 @@ __global__ __launch_bounds__(eri_tpb(EriClass<LA, LB, LC, LD, R>::NACC)) void er
     R acc[Cls::NACC];
 #pragma unroll
     for (int n = 0; n < Cls::NACC; ++n) acc[n] = (R)0;
-    prim_loop<Cls, LT, R, PREFETCH>(acc, cst(a.pp + cst(a.bra.ppo)[bi]), cst(a.bra.ppn)[bi],
+    prim_loop<Cls, LT, R, PREFETCH, -1>(acc, cst(a.pp + cst(a.bra.ppo)[bi]), cst(a.bra.ppn)[bi],
                                     a.pp + a.ket.ppo[ki], kn, sb);
 
     const cptr<double> ABp = cst(a.bra.AB + 3 * bi);
@@ -365,6 +374,259 @@ __global__ __launch_bounds__(eri_tpb(EriClass<LA, LB, LC, LD, R>::NACC)) void er
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Staged classes: the [e0|f0] set is too large for the register file, so each
+// group of bra components e makes its own pass over the primitive quartets
+// (the intermediates shared between groups are recomputed), transfers its
+// part to the ket and stores [e|cd) to the wave's workspace slot, laid out
+// [value][lane] so that every access is one coalesced 512-byte row.  The
+// epilogue then forms one (a b| row at a time from the workspace and
+// digests it.  The grid is persistent: STAGE_SLOTS waves at most, each
+// owning one workspace slot and looping over the wave tasks.
+// ---------------------------------------------------------------------------
+constexpr int STAGE_WAVES_PER_BLOCK = 4;
+
+// One wave's workspace slot through raw buffer instructions: the descriptor
+// (slot base, slot size in bytes) sits in SGPRs, the lane offset in one VGPR,
+// and value v's row offset v * 64 * sizeof(R) is a scalar immediate, so no
+// per-value 64-bit address is ever formed.  Accesses past the slot are
+// dropped by the descriptor's range check.
+template <typename R> struct WsSlot {
+    __amdgpu_buffer_rsrc_t rsrc;
+    int voff;
+    __device__ __forceinline__ WsSlot(R* base, int nvalues, int lane)
+        : rsrc(__builtin_amdgcn_make_buffer_rsrc(base, 0, nvalues * 64 * (int)sizeof(R), 0x00020000)),
+          voff(lane * (int)sizeof(R)) {}
+    __device__ __forceinline__ R ld(int v) const
+    {
+        if constexpr (sizeof(R) == 8)
+            return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(rsrc, voff, v * 64 * 8, 0));
+        else
+            return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, v * 64 * 4, 0));
+    }
+    __device__ __forceinline__ void st(int v, R x) const
+    {
+        if constexpr (sizeof(R) == 8)
+            __builtin_amdgcn_raw_buffer_store_b64(
+                __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, x), rsrc, voff, v * 64 * 8, 0);
+        else
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), rsrc, voff, v * 64 * 4, 0);
+    }
+};
+
+template <class Cls, int LT, typename R, int G, typename BraPtr>
+__device__ __forceinline__ void staged_group(BraPtr BP, int bn, const PrimPair* __restrict__ KP, int kn,
+                                             const double* __restrict__ sb, R CDx, R CDy, R CDz,
+                                             const WsSlot<R>& ws)
+{
+    constexpr int NF = Cls::NACC / Cls::NE;
+    constexpr int NG = (Cls::ge0(G + 1) - Cls::ge0(G)) * NF;
+    R acc[NG];
+#pragma unroll
+    for (int n = 0; n < NG; ++n) acc[n] = (R)0;
+    prim_loop<Cls, LT, R, false, G>(acc, BP, bn, KP, kn, sb);
+    Cls::template hrr_ket_grp<G>(acc, ws, CDx, CDy, CDz);
+}
+
+template <class Cls, int LT, typename R, typename BraPtr, int... Gs>
+__device__ __forceinline__ void staged_groups(std::integer_sequence<int, Gs...>, BraPtr BP, int bn,
+                                              const PrimPair* __restrict__ KP, int kn,
+                                              const double* __restrict__ sb, R CDx, R CDy, R CDz,
+                                              const WsSlot<R>& ws)
+{
+    (staged_group<Cls, LT, R, Gs>(BP, bn, KP, kn, sb, CDx, CDy, CDz, ws), ...);
+}
+
+template <int LA, int LB, int LC, int LD, typename R, int MODE>
+__global__ __launch_bounds__(64 * STAGE_WAVES_PER_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void eri_staged_kernel(EriArgs a)
+{
+    using Cls = EriClass<LA, LB, LC, LD, R>;
+    constexpr int TPB = 64 * STAGE_WAVES_PER_BLOCK;
+    constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB), NC = qbx_ncart(LC), ND = qbx_ncart(LD);
+    constexpr int LT = LA + LB + LC + LD;
+    constexpr int NCD = NC * ND;
+    constexpr int WSV = Cls::NE * NCD;          // workspace values per lane
+    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
+    stage_boys<TPB>(sboys, a.boysl);
+    __syncthreads();
+    const double* sb = (const double*)sboys;
+
+    const int lane = (int)threadIdx.x & 63;
+    const int slot = __builtin_amdgcn_readfirstlane((int)blockIdx.x * STAGE_WAVES_PER_BLOCK + ((int)threadIdx.x >> 6));
+    const int nslot = (int)gridDim.x * STAGE_WAVES_PER_BLOCK;
+    const WsSlot<R> ws((R*)a.ws + (size_t)slot * WSV * 64, WSV, lane);
+    const cptr<int> cpre = cst(a.cpre);
+    const size_t N = (size_t)a.N;
+
+    for (int wave = slot; wave < a.nwave; wave += nslot) {
+        int lo = 0, hi = a.nbra;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (cpre[mid] <= wave) lo = mid; else hi = mid;
+        }
+        const int bi = lo;
+        const int nk = (int)(cst(a.prefix)[bi + 1] - cst(a.prefix)[bi]);
+        const int pos = (wave - cpre[bi]) * 64 + lane;
+        const bool inrange = pos < nk;
+        const int ki = cst(a.koff)[bi] + (inrange ? pos : 0);
+        const int sA = cst(a.bra.shA)[bi], sB = cst(a.bra.shB)[bi];
+        const int sC = a.ket.shA[ki], sD = a.ket.shB[ki];
+        int kn = inrange ? a.ket.ppn[ki] : 0;
+        if (MODE == MODE_JK && a.pdmax) {
+            const double* dm = a.pdmax;
+            const int ns = a.nshell;
+            const double dab = cst(dm)[sA * ns + sB];
+            double d = fmax(fmax(dab, dm[sC * ns + sD]), fmax(dm[sA * ns + sC], dm[sA * ns + sD]));
+            d = fmax(d, fmax(dm[sB * ns + sC], dm[sB * ns + sD]));
+            if (cst(a.bra.Q)[bi] * a.ket.Q[ki] * d < a.dthr) kn = 0;
+        }
+        if (__ballot(kn > 0) == 0) continue;
+        const bool live = kn > 0;
+
+        const double* CDp = a.ket.AB + 3 * ki;
+        staged_groups<Cls, LT, R>(std::make_integer_sequence<int, Cls::NGRP>{},
+                                  cst(a.pp + cst(a.bra.ppo)[bi]), cst(a.bra.ppn)[bi],
+                                  a.pp + a.ket.ppo[ki], kn, sb, (R)CDp[0], (R)CDp[1], (R)CDp[2], ws);
+
+        const cptr<double> ABp = cst(a.bra.AB + 3 * bi);
+        const R ABx = (R)ABp[0], ABy = (R)ABp[1], ABz = (R)ABp[2];
+        const cptr<DevShell> SA = cst(a.sh + sA);
+        const cptr<DevShell> SB = cst(a.sh + sB);
+        const DevShell* SC = a.sh + sC;
+        const DevShell* SD = a.sh + sD;
+        int fc[NC], fd[ND];
+        double scc[NC], scdd[ND];
+#pragma unroll
+        for (int n = 0; n < NC; ++n) { fc[n] = SC->bf[n]; scc[n] = SC->sc[n]; }
+#pragma unroll
+        for (int n = 0; n < ND; ++n) { fd[n] = SD->bf[n]; scdd[n] = SD->sc[n]; }
+
+        if (MODE == MODE_DENSE) {
+            if (!live) continue;
+#pragma unroll 1
+            for (int iab = 0; iab < NA * NB; ++iab) {
+                const int ia = iab / NB, ib = iab - ia * NB;
+                R out[NCD];
+                Cls::hrr_bra_ab(iab, ws, out, ABx, ABy, ABz);
+                const int fa_ = SA->bf[ia], fb_ = SB->bf[ib];
+                if (fa_ < 0 || fb_ < 0) continue;
+                const double sab = SA->sc[ia] * SB->sc[ib];
+                const size_t i = fa_, j = fb_;
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) {
+                        if (fc[ic] < 0 || fd[id] < 0) continue;
+                        const size_t k = fc[ic], l = fd[id];
+                        const double v = (double)out[ic * ND + id] * sab * (scc[ic] * scdd[id]);
+                        a.eri[((i * N + j) * N + k) * N + l] = v;
+                        a.eri[((j * N + i) * N + k) * N + l] = v;
+                        a.eri[((i * N + j) * N + l) * N + k] = v;
+                        a.eri[((j * N + i) * N + l) * N + k] = v;
+                        a.eri[((k * N + l) * N + i) * N + j] = v;
+                        a.eri[((l * N + k) * N + i) * N + j] = v;
+                        a.eri[((k * N + l) * N + j) * N + i] = v;
+                        a.eri[((l * N + k) * N + j) * N + i] = v;
+                    }
+            }
+            continue;
+        }
+
+        // ---- J/K digest from the (a b| rows, in two passes that each keep
+        //      few sums live: A) J_ab, K_ac, K_ad per row; B) J_cd summed over
+        //      all rows, K_bd and K_bc summed over a ----
+        double f = live ? 1.0 : 0.0;
+        if (sA == sB) f *= 0.5;
+        if (sC == sD) f *= 0.5;
+        if (a.same && bi == ki) f *= 0.5;
+        const double fj = 4.0 * f, fk = 2.0 * f;
+#pragma unroll
+        for (int n = 0; n < NC; ++n) fc[n] = fc[n] < 0 ? 0 : fc[n];
+#pragma unroll
+        for (int n = 0; n < ND; ++n) fd[n] = fd[n] < 0 ? 0 : fd[n];
+        for (int s = 0; s < a.nset; ++s) {
+            const double* __restrict__ DJ = a.D + (size_t)s * N * N;
+            const double* __restrict__ DK = a.D + (size_t)(a.nset + s) * N * N;
+            double* Js = a.J + (size_t)s * N * N;
+            double* Ks = a.K + (size_t)s * N * N;
+#pragma unroll 1
+            for (int iab = 0; iab < NA * NB; ++iab) {
+                const int ia = iab / NB, ib = iab - ia * NB;
+                R out[NCD];
+                Cls::hrr_bra_ab(iab, ws, out, ABx, ABy, ABz);
+                const int fa_ = max(SA->bf[ia], 0), fb_ = max(SB->bf[ib], 0);
+                const double sab = SA->sc[ia] * SB->sc[ib];
+                double jab = 0.0, kac[NC], kad[ND];
+#pragma unroll
+                for (int n = 0; n < NC; ++n) kac[n] = 0.0;
+#pragma unroll
+                for (int n = 0; n < ND; ++n) kad[n] = 0.0;
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) {
+                        const double v = (double)out[ic * ND + id] * sab * (scc[ic] * scdd[id]);
+                        jab += v * DJ[fc[ic] * N + fd[id]];
+                        kac[ic] += v * DK[fb_ * N + fd[id]];
+                        kad[id] += v * DK[fb_ * N + fc[ic]];
+                    }
+                jab = wave_sum(fj * jab);
+                if (lane == 0) atomicAdd(Js + fa_ * N + fb_, jab);
+                if (live) {
+#pragma unroll
+                    for (int ic = 0; ic < NC; ++ic) atomicAdd(Ks + fa_ * N + fc[ic], fk * kac[ic]);
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) atomicAdd(Ks + fa_ * N + fd[id], fk * kad[id]);
+                }
+            }
+            double jcd[NCD];
+#pragma unroll
+            for (int n = 0; n < NCD; ++n) jcd[n] = 0.0;
+#pragma unroll 1
+            for (int ib = 0; ib < NB; ++ib) {
+                const int fb_ = max(SB->bf[ib], 0);
+                double kbd[ND], kbc[NC];
+#pragma unroll
+                for (int n = 0; n < ND; ++n) kbd[n] = 0.0;
+#pragma unroll
+                for (int n = 0; n < NC; ++n) kbc[n] = 0.0;
+#pragma unroll 1
+                for (int ia = 0; ia < NA; ++ia) {
+                    R out[NCD];
+                    Cls::hrr_bra_ab(ia * NB + ib, ws, out, ABx, ABy, ABz);
+                    const int fa_ = max(SA->bf[ia], 0);
+                    const double sab = SA->sc[ia] * SB->sc[ib];
+                    const double dab = cst(DJ)[fa_ * N + fb_];
+#pragma unroll
+                    for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                        for (int id = 0; id < ND; ++id) {
+                            const int cd = ic * ND + id;
+                            const double v = (double)out[cd] * sab * (scc[ic] * scdd[id]);
+                            jcd[cd] += v * dab;
+                            kbd[id] += v * DK[fa_ * N + fc[ic]];
+                            kbc[ic] += v * DK[fa_ * N + fd[id]];
+                        }
+                }
+                if (live) {
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) atomicAdd(Ks + fb_ * N + fd[id], fk * kbd[id]);
+#pragma unroll
+                    for (int ic = 0; ic < NC; ++ic) atomicAdd(Ks + fb_ * N + fc[ic], fk * kbc[ic]);
+                }
+            }
+            if (live) {
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) atomicAdd(Js + fc[ic] * N + fd[id], fj * jcd[ic * ND + id]);
+            }
+        }
+    }
+}
+
 // Schwarz bounds: one thread per diagonal shell pair (ab|ab).
 template <int LA, int LB, typename R>
 __global__ __launch_bounds__(256) void eri_schwarz_kernel(EriArgs a)
@@ -384,7 +646,7 @@ __global__ __launch_bounds__(256) void eri_schwarz_kernel(EriArgs a)
     for (int n = 0; n < Cls::NACC; ++n) acc[n] = (R)0;
     const PrimPair* P = a.pp + a.bra.ppo[bi];
     const int n = a.bra.ppn[bi];
-    prim_loop<Cls, LT, R, false>(acc, P, n, P, n, (const double*)sboys);
+    prim_loop<Cls, LT, R, false, -1>(acc, P, n, P, n, (const double*)sboys);
     const double* AB = a.bra.AB + 3 * bi;
     R kacc[Cls::NE * NCD];
     Cls::hrr_ket(acc, kacc, (R)AB[0], (R)AB[1], (R)AB[2]);
@@ -414,6 +676,16 @@ void launch_eri(const EriArgs& a, hipStream_t s)
         if (a.total <= 0) return;
         const int64_t blocks = (a.total + 255) / 256;
         hipLaunchKernelGGL((eri_schwarz_kernel<LA, LB, R>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+        QBX_HIP(hipGetLastError());
+    } else if constexpr (EriClass<LA, LB, LC, LD, R>::STAGED) {
+        if (a.nwave <= 0) return;
+        if (!a.ws) throw QbxError(QBX_EINTERNAL, "staged ERI class launched without a workspace");
+        const int slots = a.nwave < a.ws_slots ? a.nwave : a.ws_slots;
+        const int blocks = (slots + STAGE_WAVES_PER_BLOCK - 1) / STAGE_WAVES_PER_BLOCK;
+        if (blocks * STAGE_WAVES_PER_BLOCK > a.ws_slots)
+            throw QbxError(QBX_EINTERNAL, "staged ERI grid exceeds its workspace");
+        hipLaunchKernelGGL((eri_staged_kernel<LA, LB, LC, LD, R, MODE>), dim3((unsigned)blocks),
+                           dim3(64 * STAGE_WAVES_PER_BLOCK), 0, s, a);
         QBX_HIP(hipGetLastError());
     } else {
         if (a.nwave <= 0) return;

@@ -44,6 +44,10 @@ struct EriArgs {
     int nshell;
     double dthr;
     void* scratch_stream;  // main stream: kernels with scratch are launched there
+    // staged classes: workspace of ws_slots wave slots, each
+    // EriClass::NE * NCD * 64 values of the kernel's arithmetic type
+    void* ws;
+    int ws_slots;
 };
 
 

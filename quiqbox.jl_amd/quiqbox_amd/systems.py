@@ -11,13 +11,17 @@
                               with 2-primitive d, exponents log-uniform in [0.2, 5]) and
                               composition 50% H / 30% C / 20% O in seeded random order:
                               592 functions and 624 electrons at 160 atoms.
+  water_box(n_mol, seed, basis)
+                              closed-shell water cluster at liquid density (random
+                              orientations), random shells or a library basis set.
   h_cluster_sp(n_atoms, seed) random H cluster with floating 4-primitive s and p shells.
 """
 import math
 
 import numpy as np
 
-from .basis import CompositeOrb, PrimitiveOrb, genGaussTypeOrb, get3DimPGTOrbNormFactor, subshellXYZs
+from .basis import (CompositeOrb, PrimitiveOrb, genGaussTypeOrb, genGaussTypeOrbSeq, get3DimPGTOrbNormFactor,
+                    subshellXYZs)
 from .particles import NuclearCluster, getCharge
 
 H10_XPN = (3.42525091, 0.62391373, 0.16885540)
@@ -82,6 +86,57 @@ def random_cluster(n_atoms=64, seed=0, with_d=False, box=20.0, dmin=1.8):
                 xp = np.exp(rng.uniform(math.log(0.1), math.log(50.0), size=3))
                 co = rng.uniform(0.1, 1.0, size=3)
                 bs += _shell(c, 0 if sh == "s" else 1, xp, co)
+    return nuc, bs
+
+
+def _random_shells(rng, elem, center):
+    """Element-dependent random shells of random_cluster(with_d=True)."""
+    bs = []
+    for sh in {"H": ["s"], "C": ["s", "p"], "O": ["s", "p", "d"]}[elem]:
+        if sh == "d":
+            xp = np.exp(rng.uniform(math.log(0.2), math.log(5.0), size=2))
+            co = rng.uniform(0.1, 1.0, size=2)
+            bs += _shell(center, 2, xp, co)
+        else:
+            xp = np.exp(rng.uniform(math.log(0.1), math.log(50.0), size=3))
+            co = rng.uniform(0.1, 1.0, size=3)
+            bs += _shell(center, 0 if sh == "s" else 1, xp, co)
+    return bs
+
+
+def water_box(n_mol=50, seed=0, density=1.0, doo=5.0, basis=None):
+    """Closed-shell water cluster at liquid density: rigid H2O (O-H 1.809 bohr,
+    H-O-H 104.5 deg) in random orientations, oxygens at least `doo` bohr apart
+    in a cube sized for `density` g/cm^3.  With basis=None every atom carries
+    the element-dependent random shells of random_cluster(with_d=True) (O: s,
+    p, d; H: s), so 50 molecules give 150 atoms and 600 functions; with a
+    library name (e.g. "cc-pVDZ": 25 Cartesian functions per molecule, 24
+    molecules give 600) the standard basis set."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    # 18.015 g/mol, 1 bohr = 0.529177 A
+    vol_a3 = n_mol * 18.015 / (density * 0.6022140857)
+    box = vol_a3 ** (1.0 / 3.0) / 0.529177210903
+    oxy = _place_atoms(rng, n_mol, box, doo)
+    roh, half = 1.809, math.radians(104.5) / 2.0
+    elems, coords = [], []
+    for o in oxy:
+        # random rotation from a normalized quaternion
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        w, x, y, z = q
+        Rm = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                       [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                       [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+        h1 = Rm @ np.array([roh * math.sin(half), 0.0, roh * math.cos(half)])
+        h2 = Rm @ np.array([-roh * math.sin(half), 0.0, roh * math.cos(half)])
+        o = np.asarray(o)
+        for e, c in (("O", o), ("H", o + h1), ("H", o + h2)):
+            elems.append(e)
+            coords.append(tuple(float(v) for v in c))
+    nuc = NuclearCluster(elems, coords, pairwiseSort=False)
+    bs = []
+    for e, c in zip(elems, coords):
+        bs += _random_shells(rng, e, c) if basis is None else genGaussTypeOrbSeq(c, e, basis)
     return nuc, bs
 
 
