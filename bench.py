@@ -236,6 +236,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--serial", action="store_true",
+                    help="run the class kernels of each build one after another on one stream (for profiling)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -268,6 +270,8 @@ def main():
     log("screened: %d shell quartets, %d unique Cartesian quartets, %d primitive shell quartets"
         % (stats["shell_quartets"].sum(), n_cart, stats["prim_quartets"].sum()))
 
+    if args.serial:
+        gb.ctx.set_timing(True)
     for _ in range(args.warmup):
         gb.jk_device(dDJ, dDK, dG, 1, thr)
     hip.sync()
