@@ -169,6 +169,37 @@ def run_scf(nuc, gb, precision, thr=1e-8, incremental=False):
     return res, time.perf_counter() - t0
 
 
+def gradient_section(steps):
+    """Basis-parameter gradient on the 32-atom H cluster with floating
+    4-primitive s and p shells (256 exponents, 256 coefficients, 96 centre
+    coordinates): one adjoint gradient (SCF + kernels) timed alone, then
+    `steps` gradient-descent steps of basis optimization."""
+    import quiqbox_amd as qb
+    from quiqbox_amd.gradient import ParamBasis, optimize_basis, rhf_gradient
+    from quiqbox_amd.systems import h_cluster_sp
+    nuc, _ = h_cluster_sp(32, 0)
+    pb = ParamBasis.random_h_cluster(32, 0)
+    gb = qb.GPUBasis(pb.orbitals())
+    cfg = qb.HFconfig(initial="CoreH", strategy=qb.SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-10)), maxStep=200)
+    t0 = time.perf_counter()
+    res = qb.runHartreeFock(nuc, gb, cfg, printInfo=False)
+    t_scf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    E, g, _ = rhf_gradient(nuc, pb, res=res, gpu_basis=gb)
+    t_grad = time.perf_counter() - t0
+    log("gradient: %d parameters, SCF %.3f s, adjoint gradient %.3f s, |g| %.3e" % (pb.n_theta, t_scf, t_grad,
+                                                                                     np.linalg.norm(g)))
+    t0 = time.perf_counter()
+    _, hist = optimize_basis(nuc, pb, steps=steps, lr=0.02, config=cfg)
+    t_opt = time.perf_counter() - t0
+    log("basis optimization: %d steps in %.2f s, E %.8f -> %.8f" % (steps, t_opt, hist[0], hist[-1]))
+    return {"workload": "32 H atoms, floating 4-primitive s and p shells (seed 0), 128 functions, %d parameters"
+                        % pb.n_theta,
+            "scf_s": t_scf, "gradient_s": t_grad, "grad_norm": float(np.linalg.norm(g)),
+            "opt_steps": steps, "opt_wall_s": t_opt, "opt_s_per_step": t_opt / steps,
+            "energy_first": hist[0], "energy_last": hist[-1]}
+
+
 def rhf_section(hip, cpu_seconds, no_cpu):
     """RHF wall time to |dE| < 1e-8 Eh on the 600-function water cluster: plain
     FP64 and mixed precision + incremental builds, plus that system's FP64 Fock
@@ -236,6 +267,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--grad-steps", type=int, default=100,
+                    help="basis-optimization steps of the gradient section (0: skip it)")
     ap.add_argument("--serial", action="store_true",
                     help="run the class kernels of each build one after another on one stream (for profiling)")
     args = ap.parse_args()
@@ -339,6 +372,8 @@ def main():
         hip.free(p)
     if not args.no_scf:
         out["rhf"] = rhf_section(hip, min(args.cpu_seconds, 8.0), args.no_cpu)
+        if args.grad_steps > 0:
+            out["gradient"] = gradient_section(args.grad_steps)
     print(json.dumps(out), flush=True)
 
 

@@ -126,6 +126,62 @@ int  qbx_set_mixed_cut(qbx_ctx* ctx, double cut);
    No reference counterpart: the reference contracts a stored tensor. */
 int  qbx_set_density_screening(qbx_ctx* ctx, double thr);
 
+/* Shell table of a basis (the shells found by qbx_basis_create: functions with a
+   common centre and exponent list whose weights are proportional).  Shell s has
+   angular momentum shell_L[s] and shell_nprim[s] radial primitives
+   radial_xpn/radial_coef[shell_poff[s] + k] (the coefficients of its first
+   component's weights); component c (Quiqbox order) is basis function
+   shell_bf[6 s + c] (-1 if absent) with weight ratio shell_sc[6 s + c].
+   Arrays: n_shell entries (x6 for bf/sc, x3 for the centres), n_radial for the
+   radial arrays; NULL entries skipped.  No reference counterpart: it exposes
+   how the basis maps onto the kernels' shells, which the gradient reports in. */
+int  qbx_basis_shells(const qbx_basis* basis, int32_t* n_shell, int32_t* n_radial, int32_t* shell_L,
+                      int32_t* shell_nprim, int32_t* shell_poff, int32_t* shell_bf, double* shell_sc,
+                      double* shell_cen, double* radial_xpn, double* radial_coef);
+
+/* Gradient of the RHF energy with respect to the basis parameters, s and p
+   shells with at most 8 radial primitives (QBX_EUNSUP otherwise).  Inputs: the
+   converged total density Dt = 2 C_occ C_occ^T and the energy-weighted density
+   W = 2 C_occ diag(eps_occ) C_occ^T (n_bf x n_bf), the nuclei (charges Z, R
+   3*n_nuc, fixed), the Schwarz threshold of the ERI task lists.  Outputs, from
+   E = sum D (T+V) - sum W S + 1/2 sum Gamma (mu nu|lam sig) with every basis
+   function's weights held fixed:
+     g_xpn[n_radial]  dE/d(exponent) of each radial primitive,
+     g_lin[n_radial]  c dE/dc for its coefficient c (all components scaled alike),
+     g_cen[3*n_shell] dE/d(centre) of each shell.
+   The reference has no gradient (Quiqbox v0.6.3 offers the parameter boxes of
+   Parameters.jl:14-43 but no derivative of the energy); the host chains these
+   through the normalization to the user's parameters. */
+int  qbx_rhf_gradient(qbx_ctx* ctx, qbx_basis* basis, const double* Dt, const double* W, int32_t n_nuc,
+                      const double* Z, const double* R, double schwarz_thr, double* g_xpn, double* g_lin,
+                      double* g_cen);
+
+/* ---- BF16 matrix-core kernels of the learned initial guess --------------
+   No reference counterpart: a pair-feature MLP that predicts the initial SCF
+   density (the reference's guesses are CoreH/GWH/SAD, HartreeFock.jl:215-293).
+   Device pointers, enqueued on `stream` (NULL: default stream), no sync.
+   Matrices row-major; BF16 as raw 16-bit patterns.
+
+   C[M x N] (FP32, row stride ldc) = alpha op(A) op(B) + beta C, op(A) = A
+   (M x K, stride lda) or A^T (A stored K x M), likewise B; FP32 accumulation
+   on the MFMA units.  epilogue 0: plain; 1: C = relu(alpha AB + bias[n]) and
+   its BF16 copy Cb; 2: C = alpha AB + bias[n].  Epilogues 1-2 take
+   untransposed operands. */
+int  qbx_gemm_bf16(int32_t trans_a, int32_t trans_b, int32_t M, int32_t N, int32_t K, float alpha,
+                   const void* A, int32_t lda, const void* B, int32_t ldb, float beta, float* C, int32_t ldc,
+                   int32_t epilogue, const float* bias, void* Cb, void* stream);
+/* op 0: out0 (BF16) = x (FP32) converted; op 1: ReLU backward, out1 (FP32) and
+   out0 (BF16) = x (FP32 dH) where y (BF16 H) > 0, else 0. */
+int  qbx_mlp_elementwise(int32_t op, int64_t n, const void* x, const void* y, void* out0, void* out1, void* stream);
+/* weighted squared error: loss += sum w_r m_c (Y - T)^2 scale, dY = 2 w_r m_c (Y - T) scale (FP32 and BF16) */
+int  qbx_mlp_mse(int32_t rows, int32_t cols, const float* Y, const float* T, const float* row_w,
+                 const float* col_mask, float scale, float* dY, void* dYb, float* loss, void* stream);
+/* out[c] = sum_r X[r][c] */
+int  qbx_mlp_colsum(int32_t rows, int32_t cols, const float* X, float* out, void* stream);
+/* Adam step `step` (1-based) on FP32 weights w with moments m, v and gradient g; BF16 copy to wb if non-NULL */
+int  qbx_mlp_adam(int64_t n, float* w, float* m, float* v, const float* g, void* wb, float lr, float b1, float b2,
+                  float eps, int32_t step, void* stream);
+
 /* Last error message of the calling thread; returns its full length. */
 int  qbx_last_error(char* buf, size_t len);
 

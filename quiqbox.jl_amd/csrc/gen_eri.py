@@ -36,6 +36,16 @@ group's pass over the primitive quartets.  hrr_bra_ab then forms one
 (a b | c d) row for all cd from the workspace, the bra transfer written out
 as polynomials in (A-B):  (a,b| = sum_e P_e(A-B) (e,0|.
 
+For the RHF energy gradient (classes with s and p shells) it also emits
+EriAdj<la, lb, lc, ld>: the reverse-mode (adjoint) form of the same
+recurrences, written out statement by statement from the recurrence graph:
+  hrr_adj   seeds on the contracted (ab|cd) -> seeds on the contracted
+            [e0|f0] plus the derivatives with respect to A-B and C-D;
+  vrr_adj   per primitive quartet, the forward recurrence, then its adjoint
+            with the [e0|f0] seeds -> derivatives with respect to the Boys
+            values F_m and the geometric inputs (PA, WP, QC, WQ, 1/2p, 1/2q,
+            1/2(p+q), q/(p+q), p/(p+q)), and the seeded sum itself.
+
 The generator also writes eri_flops.json: arithmetic operations per primitive
 quartet (VRR + Boys seed) and per contracted quartet (HRR) for each class,
 counted from the emitted expressions (add/sub/mul = 1, fma pattern a*b+c
@@ -47,6 +57,8 @@ import os
 import sys
 
 LMAX = 2
+# classes with an adjoint (gradient) kernel: every shell s or p
+GRAD_LMAX = 1
 # classes evaluated in groups of bra components through a workspace
 STAGED = {(2, 1, 1, 1), (2, 1, 2, 1), (2, 2, 1, 1), (2, 2, 2, 0), (2, 2, 2, 1), (2, 2, 2, 2)}
 STAGE_ACC = 32     # accumulators per group
@@ -298,6 +310,213 @@ def gen_staged(la, lb, lc, ld, es, fs, eidx, fidx, C_, D_):
     return code, flops_vrr_exec, flops_bra
 
 
+VRR_INPUTS = ["PAx", "PAy", "PAz", "WPx", "WPy", "WPz", "QCx", "QCy", "QCz",
+              "WQx", "WQy", "WQz", "inv2p", "inv2q", "inv2pq", "rho_p", "rho_q"]
+
+
+def vrr_dag(targets):
+    """The vertical recurrence of gen_vrr as a graph: nodes [(name, terms)] in
+    evaluation order and {(e, f): name} for the m = 0 targets.  Terms:
+      ("lin", coef, x)            coef * x
+      ("ho", k, inv, rho, c, d)   k * inv * (c - rho * d)
+      ("tr", k, g)                k * inv2pq * g"""
+    nodes = []
+    memo = {}
+
+    def V(e, f, m):
+        key = (e, f, m)
+        if key in memo:
+            return memo[key]
+        if sum(e) == 0 and sum(f) == 0:
+            memo[key] = "F[%d]" % m
+            return memo[key]
+        terms = []
+        if sum(f) > 0:
+            i = pick_axis(f)
+            f1 = sub1(f, i)
+            terms.append(("lin", "QC%s" % AX[i], V(e, f1, m)))
+            terms.append(("lin", "WQ%s" % AX[i], V(e, f1, m + 1)))
+            if f1[i] > 0:
+                f2 = sub1(f1, i)
+                terms.append(("ho", f1[i], "inv2q", "rho_q", V(e, f2, m), V(e, f2, m + 1)))
+            if e[i] > 0:
+                terms.append(("tr", e[i], V(sub1(e, i), f1, m + 1)))
+        else:
+            i = pick_axis(e)
+            e1 = sub1(e, i)
+            terms.append(("lin", "PA%s" % AX[i], V(e1, f, m)))
+            terms.append(("lin", "WP%s" % AX[i], V(e1, f, m + 1)))
+            if e1[i] > 0:
+                e2 = sub1(e1, i)
+                terms.append(("ho", e1[i], "inv2p", "rho_p", V(e2, f, m), V(e2, f, m + 1)))
+        name = "v%d" % (len(nodes) + 1)
+        nodes.append((name, terms))
+        memo[key] = name
+        return name
+
+    out = {}
+    for (e, f) in targets:
+        out[(e, f)] = V(e, f, 0)
+    return nodes, out
+
+
+def _adj(x):
+    if x.startswith("F["):
+        return "Fb[%s" % x[2:]
+    if x.startswith("acc["):
+        return "accb[%s" % x[4:]
+    if x in VRR_INPUTS or x in ("ABx", "ABy", "ABz", "CDx", "CDy", "CDz"):
+        return "b_" + x
+    return "b" + x
+
+
+def gen_adjoint(la, lb, lc, ld):
+    """EriAdj<la, lb, lc, ld> (see the module docstring) and its flop counts."""
+    es = [e for L in range(la, la + lb + 1) for e in cart(L)]
+    fs = [f for L in range(lc, lc + ld + 1) for f in cart(L)]
+    eidx = {e: n for n, e in enumerate(es)}
+    fidx = {f: n for n, f in enumerate(fs)}
+    nF = len(fs)
+    targets = [(e, f) for e in es for f in fs]
+    nodes, out = vrr_dag(targets)
+    Lt = la + lb + lc + ld
+    fwd, rev = [], []
+    flops = 0
+    for name, terms in nodes:
+        ex = []
+        for t in terms:
+            if t[0] == "lin":
+                ex.append("%s * %s" % (t[1], t[2]))
+            elif t[0] == "ho":
+                ex.append("%d.0 * %s * (%s - %s * %s)" % (t[1], t[2], t[4], t[3], t[5]))
+            else:
+                ex.append("%d.0 * inv2pq * %s" % (t[1], t[2]))
+        fwd.append("const double %s = %s;" % (name, " + ".join(ex)))
+    decl = ["double %s = 0.0;" % _adj(n) for n, _ in nodes]
+    decl += ["double b_%s = 0.0;" % x for x in VRR_INPUTS]
+    seed = []
+    lsum = []
+    for k, t in enumerate(targets):
+        seed.append("%s += g[%d];" % (_adj(out[t]), k))
+        lsum.append("g[%d] * %s" % (k, out[t]))
+    for name, terms in reversed(nodes):
+        b = _adj(name)
+        for t in terms:
+            if t[0] == "lin":
+                rev.append("%s += %s * %s;" % (_adj(t[2]), t[1], b))
+                rev.append("b_%s += %s * %s;" % (t[1], t[2], b))
+                flops += 4
+            elif t[0] == "ho":
+                k, inv, rho, c, d = t[1:]
+                rev.append("%s += %d.0 * %s * %s;" % (_adj(c), k, inv, b))
+                rev.append("%s -= %d.0 * %s * %s * %s;" % (_adj(d), k, inv, rho, b))
+                rev.append("b_%s += %d.0 * (%s - %s * %s) * %s;" % (inv, k, c, rho, d, b))
+                rev.append("b_%s -= %d.0 * %s * %s * %s;" % (rho, k, inv, d, b))
+                flops += 14
+            else:
+                rev.append("%s += %d.0 * inv2pq * %s;" % (_adj(t[2]), t[1], b))
+                rev.append("b_inv2pq += %d.0 * %s * %s;" % (t[1], t[2], b))
+                flops += 6
+    code = []
+    code.append("template <> struct EriAdj<%d, %d, %d, %d> {" % (la, lb, lc, ld))
+    code.append("    static constexpr int NACC = %d;" % len(targets))
+    code.append("    // forward recurrence of one primitive quartet, then its adjoint with seeds")
+    code.append("    // g[NACC] on [e0|f0]: Fb[m] = d/dF_m, bin[] = d/d(%s)," % ", ".join(VRR_INPUTS[:4]) + " ...")
+    code.append("    // in the order PA, WP, QC, WQ (x, y, z each), inv2p, inv2q, inv2pq, rho_p, rho_q;")
+    code.append("    // returns sum_k g[k] [e0|f0]_k")
+    code.append("    static __device__ __forceinline__ double vrr_adj(const double* __restrict__ F, "
+                "const double* __restrict__ g,")
+    code.append("        double PAx, double PAy, double PAz, double WPx, double WPy, double WPz,")
+    code.append("        double QCx, double QCy, double QCz, double WQx, double WQy, double WQz,")
+    code.append("        double inv2p, double inv2q, double inv2pq, double rho_p, double rho_q,")
+    code.append("        double* __restrict__ Fb, double* __restrict__ bin) {")
+    code += ["        " + l for l in fwd]
+    code.append("#pragma unroll")
+    code.append("        for (int m = 0; m <= %d; ++m) Fb[m] = 0.0;" % Lt)
+    code += ["        " + l for l in decl + seed + rev]
+    code.append("        " + " ".join("bin[%d] = b_%s;" % (k, x) for k, x in enumerate(VRR_INPUTS)))
+    code.append("        return %s;" % " + ".join(lsum))
+    code.append("    }")
+
+    # full horizontal recurrence (ket, then bra) as a graph of n = x + coef * y
+    C_, D_, A_, B_ = cart(lc), cart(ld), cart(la), cart(lb)
+    nC, nD, nB = len(C_), len(D_), len(B_)
+    NCD = nC * nD
+    hn = []
+    kmemo, bmemo = {}, {}
+
+    def K(e, c, d):
+        key = (e, c, d)
+        if key in kmemo:
+            return kmemo[key]
+        if sum(d) == 0:
+            r = "acc[%d]" % (eidx[e] * nF + fidx[c])
+            kmemo[key] = r
+            return r
+        i = pick_axis(d)
+        d1 = sub1(d, i)
+        x = K(e, add1(c, i), d1)
+        y = K(e, c, d1)
+        name = "k%d" % (len(hn) + 1)
+        hn.append((name, x, "CD%s" % AX[i], y))
+        kmemo[key] = name
+        return name
+
+    def Bv(a, b, c, d):
+        key = (a, b, c, d)
+        if key in bmemo:
+            return bmemo[key]
+        if sum(b) == 0:
+            r = K(a, c, d)
+            bmemo[key] = r
+            return r
+        i = pick_axis(b)
+        b1 = sub1(b, i)
+        x = Bv(add1(a, i), b1, c, d)
+        y = Bv(a, b1, c, d)
+        name = "h%d" % (len(hn) + 1)
+        hn.append((name, x, "AB%s" % AX[i], y))
+        bmemo[key] = name
+        return name
+
+    outs = []
+    for a0 in A_:
+        for b0 in B_:
+            for c0 in C_:
+                for d0 in D_:
+                    outs.append(Bv(a0, b0, c0, d0))
+    hf = ["const double %s = %s + %s * %s;" % (n, x, c, y) for (n, x, c, y) in hn]
+    hdecl = ["double %s = 0.0;" % _adj(n) for (n, _, _, _) in hn]
+    hdecl += ["double b_%s = 0.0;" % x for x in ("ABx", "ABy", "ABz", "CDx", "CDy", "CDz")]
+    hseed = ["%s += G[%d];" % (_adj(o), k) for k, o in enumerate(outs)]
+    hrev = []
+    hflops = 0
+    for (n, x, c, y) in reversed(hn):
+        b = _adj(n)
+        hrev.append("%s += %s;" % (_adj(x), b))
+        hrev.append("%s += %s * %s;" % (_adj(y), c, b))
+        hrev.append("b_%s += %s * %s;" % (c, y, b))
+        hflops += 5
+    code.append("    static constexpr int NINT = %d;" % len(outs))
+    code.append("    // adjoint of the horizontal recurrence: seeds G[NINT] on the contracted (ab|cd)")
+    code.append("    // (index ((ia * NB + ib) * NC + ic) * ND + id) -> accb[NACC] on the contracted [e0|f0],")
+    code.append("    // bAB[3], bCD[3] = derivatives with respect to A-B and C-D; returns sum_k G[k] (ab|cd)_k")
+    code.append("    static __device__ __forceinline__ double hrr_adj(const double* __restrict__ acc, "
+                "const double* __restrict__ G,")
+    code.append("        double* __restrict__ accb, double* __restrict__ bAB, double* __restrict__ bCD,")
+    code.append("        double ABx, double ABy, double ABz, double CDx, double CDy, double CDz) {")
+    code += ["        " + l for l in hf]
+    code.append("#pragma unroll")
+    code.append("        for (int n = 0; n < NACC; ++n) accb[n] = 0.0;")
+    code += ["        " + l for l in hdecl + hseed + hrev]
+    code.append("        bAB[0] = b_ABx; bAB[1] = b_ABy; bAB[2] = b_ABz;")
+    code.append("        bCD[0] = b_CDx; bCD[1] = b_CDy; bCD[2] = b_CDz;")
+    code.append("        return %s;" % " + ".join("G[%d] * %s" % (k, o) for k, o in enumerate(outs)))
+    code.append("    }")
+    code.append("};")
+    return "\n".join(code), flops, hflops
+
+
 def stage_ws(la, lb, lc, ld):
     """[e|cd) values per lane that a staged class keeps in its workspace."""
     if (la, lb, lc, ld) not in STAGED:
@@ -457,6 +676,18 @@ def main(outdir):
         variants = [("double", "MODE_JK", "jk64"), ("float", "MODE_JK", "jk32"), ("double", "MODE_DENSE", "dense")]
         if (la, lb) == (lc, ld):
             variants.append(("double", "MODE_SCHWARZ", "schwarz"))
+        if max(la, lb, lc, ld) <= GRAD_LMAX:
+            acode, af, ah = gen_adjoint(la, lb, lc, ld)
+            flops[tag]["adjoint_vrr_per_prim_quartet"] = af
+            flops[tag]["adjoint_hrr_per_quartet"] = ah
+            tu = ["// Generated by gen_eri.py -- do not edit.",
+                  "// ERI class (%d %d | %d %d): adjoint recurrences and gradient kernel instantiation."
+                  % (la, lb, lc, ld),
+                  '#include "qbx_grad.h"', "", code, "", acode, "", "namespace qbx {",
+                  "template void launch_eri_grad<%d, %d, %d, %d>(const EriArgs&, hipStream_t);" % (la, lb, lc, ld),
+                  "}  // namespace qbx"]
+            with open(os.path.join(outdir, "eri_%s_grad.hip" % tag), "w") as f:
+                f.write("\n".join(tu) + "\n")
         for R, mode, vtag in variants:
             tu = ["// Generated by gen_eri.py -- do not edit.",
                   "// ERI class (%d %d | %d %d), %s %s: recurrence code and kernel instantiation."
@@ -476,6 +707,9 @@ def main(outdir):
                  len(pcl), ", ".join("{%d, %d}" % c for c in pcl)),
              "static const int qbx_class_l[%d][4] = {%s};" % (
                  len(classes), ", ".join("{%d, %d, %d, %d}" % c for c in classes)),
+             "// classes with a gradient kernel (all shells s or p)",
+             "static const int qbx_class_grad[%d] = {%s};" % (
+                 len(classes), ", ".join("1" if max(c) <= GRAD_LMAX else "0" for c in classes)),
              "// workspace values per lane of the staged classes (0: not staged)",
              "static const int qbx_class_ws[%d] = {%s};" % (
                  len(classes), ", ".join(str(stage_ws(*c)) for c in classes)),

@@ -38,28 +38,7 @@ using namespace qbx;
 namespace {
 using namespace qbx;
 
-// ---------------------------------------------------------------------------
-// errors
-// ---------------------------------------------------------------------------
-thread_local std::string g_last_error;
 using qbx::QbxError;
-
-template <typename F> int guarded(F&& f)
-{
-    try {
-        f();
-        return QBX_OK;
-    } catch (const QbxError& e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return QBX_ENOMEM;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return QBX_EINTERNAL;
-    }
-}
 
 template <typename T> struct DevBuf {
     T* p = nullptr;
@@ -153,8 +132,10 @@ struct qbx_basis {
     std::vector<HostShell> shells;
     std::vector<double> pexp, pcoef;
     std::vector<PrimPair> pp;
+    std::vector<int> ppk;           // radial indices of each primitive pair: k_a | k_b << 16
     PairClass pc[QBX_N_PAIR_CLASSES];
     DevBuf<PrimPair> d_pp;
+    DevBuf<int> d_ppk, d_poff;
     DevBuf<DevShell> d_sh;
     // all shell pairs for the one-electron kernels (index into pc lists)
     std::vector<int2> p1;       // (class, index)
@@ -239,16 +220,17 @@ struct OneArgs {
 // Nuclear-attraction primitive value for one component pair and one point C:
 // the Obara-Saika one-body scheme axis by axis with the Boys auxiliary order
 // carried from axis to axis (the scheme of GaussianOrbitals.jl:478-522).
-__device__ double coulomb1_comp(const int* la, const int* lb, const double* PA, const double* PC,
-                                const double* AB, double inv2p, const double* F, int Ltot)
+template <int NMAX>
+__device__ double coulomb1_comp_t(const int* la, const int* lb, const double* PA, const double* PC,
+                                  const double* AB, double inv2p, const double* F, int Ltot)
 {
-    double vals[2 * QBX_MAXL + 1];
+    double vals[NMAX + 1];
     for (int m = 0; m <= Ltot; ++m) vals[m] = F[m];
     int nUp = Ltot;
     for (int t = 0; t < 3; ++t) {
         const int a = la[t], b = lb[t], S = a + b;
         if (S == 0) continue;
-        double T[2 * QBX_MAXL + 1][2 * QBX_MAXL + 1];
+        double T[NMAX + 1][NMAX + 1];
         for (int j = 0; j <= nUp; ++j) T[0][j] = vals[j];
         for (int i = 0; i < S; ++i)
             for (int j = 0; j <= nUp - i - 1; ++j) {
@@ -257,7 +239,7 @@ __device__ double coulomb1_comp(const int* la, const int* lb, const double* PA, 
                 T[i + 1][j] = v;
             }
         for (int m = 0; m <= nUp - S; ++m) {
-            double h[2 * QBX_MAXL + 1];
+            double h[NMAX + 1];
             for (int i = a; i <= S; ++i) h[i] = T[i][m];
             for (int s = 0; s < b; ++s)
                 for (int k = a; k < a + b - s; ++k) h[k] = h[k + 1] + AB[t] * h[k];
@@ -266,6 +248,141 @@ __device__ double coulomb1_comp(const int* la, const int* lb, const double* PA, 
         nUp -= S;
     }
     return vals[0];
+}
+
+__device__ double coulomb1_comp(const int* la, const int* lb, const double* PA, const double* PC,
+                                const double* AB, double inv2p, const double* F, int Ltot)
+{
+    return coulomb1_comp_t<2 * QBX_MAXL>(la, lb, PA, PC, AB, inv2p, F, Ltot);
+}
+
+// ---------------------------------------------------------------------------
+// kernel: one-electron part of the RHF energy gradient
+// ---------------------------------------------------------------------------
+struct OneGradArgs {
+    const DevShell* sh;
+    const PrimPair* pp;
+    const int* ppk;
+    const int* pA;
+    const int* pB;
+    const int* po;
+    const int* pn;
+    int npair;
+    int N;
+    const double* boys;
+    const double* bexp;
+    int nnuc;
+    const double* Z;
+    const double* R;
+    const double* Dt;      // total density
+    const double* W;       // energy-weighted density
+    const int* poff;
+    double* g_xpn;
+    double* g_lin;
+    double* g_cen;
+};
+
+constexpr int kGradPairPrims = QBX_GRAD_MAXP * QBX_GRAD_MAXP;   // primitive pairs per shell pair in the 1e gradient grid
+
+// D (T + V) - W S for one primitive pair and Cartesian components la, lb of
+// the two primitives (any angular momentum up to 2 above the shells').
+__device__ double one_e_energy(const int* la, const int* lb, const PrimPair& P, const double* AB,
+                               const OneGradArgs& a, double wd, double ww)
+{
+    const double PA[3] = {P.PAx, P.PAy, P.PAz};
+    const double PB[3] = {P.PAx + AB[0], P.PAy + AB[1], P.PAz + AB[2]};
+    double S[3][QBX_OV_MAX][QBX_OV_MAX];
+    for (int t = 0; t < 3; ++t) qbx_ov1d(PA[t], PB[t], P.inv2p, la[t], lb[t] + 2, S[t]);
+    const double sx = S[0][la[0]][lb[0]], sy = S[1][la[1]][lb[1]], sz = S[2][la[2]][lb[2]];
+    const double b = P.pad;
+    double d2[3];
+    for (int t = 0; t < 3; ++t) {
+        const int j = lb[t];
+        double r = -2.0 * b * (2 * j + 1) * S[t][la[t]][j] + 4.0 * b * b * S[t][la[t]][j + 2];
+        if (j > 1) r += (double)(j * (j - 1)) * S[t][la[t]][j - 2];
+        d2[t] = r;
+    }
+    const double pre = 5.568327996831707845 * P.Kp * rsqrt(P.p);
+    const double ovl = pre * sx * sy * sz;
+    const double kin = pre * -0.5 * (d2[0] * sy * sz + sx * d2[1] * sz + sx * sy * d2[2]);
+    double nuc = 0.0;
+    if (wd != 0.0) {
+        const int Lt = la[0] + la[1] + la[2] + lb[0] + lb[1] + lb[2];
+        for (int n = 0; n < a.nnuc; ++n) {
+            const double PC[3] = {P.Px - a.R[3 * n], P.Py - a.R[3 * n + 1], P.Pz - a.R[3 * n + 2]};
+            const double T = P.p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+            double F[2 * QBX_MAXL + 5];
+            qbx_boys_rt(Lt, T, F, a.boys, a.bexp);
+            nuc += a.Z[n] * coulomb1_comp_t<2 * QBX_MAXL + 4>(la, lb, PA, PC, AB, P.inv2p, F, Lt);
+        }
+        nuc *= -2.0 * kPi * P.Kp;
+    }
+    return wd * (kin + nuc) - ww * ovl;
+}
+
+// One thread per (shell pair, primitive pair).  With I(a, b) an integral over
+// the primitives' Cartesian components a, b (weights included):
+//   dI/dA_i = 2 alpha I(a + 1_i, b) - a_i I(a - 1_i, b),
+//   dI/dalpha = -sum_i I(a + 2_i, b)   (d/dalpha exp(-alpha r_A^2) = -r_A^2 exp(...)),
+// likewise for B, beta; and c dI/dc = I for either primitive.
+__global__ __launch_bounds__(256) void one_e_grad_kernel(OneGradArgs a)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int pair = gid / kGradPairPrims, u = gid % kGradPairPrims;
+    if (pair >= a.npair || u >= a.pn[pair]) return;
+    const int sA = a.pA[pair], sB = a.pB[pair];
+    const DevShell A = a.sh[sA];
+    const DevShell B = a.sh[sB];
+    const PrimPair P = a.pp[a.po[pair] + u];
+    const int kab = a.ppk[a.po[pair] + u];
+    const int ka = kab & 0xffff, kb = kab >> 16;
+    const double alpha = P.p - P.pad, beta = P.pad;
+    const double AB[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
+    const double fac = sA == sB ? 1.0 : 2.0;
+    const size_t N = (size_t)a.N;
+    double gxa = 0.0, gxb = 0.0, gl = 0.0, gAc[3] = {0.0, 0.0, 0.0}, gBc[3] = {0.0, 0.0, 0.0};
+    for (int ca = 0; ca < qbx_ncart(A.L); ++ca)
+        for (int cb = 0; cb < qbx_ncart(B.L); ++cb) {
+            const int ia = A.bf[ca], ib = B.bf[cb];
+            if (ia < 0 || ib < 0) continue;
+            int la[3], lb[3];
+            qbx_cart(A.L, ca, la);
+            qbx_cart(B.L, cb, lb);
+            const double sc = fac * A.sc[ca] * B.sc[cb];
+            const double wd = sc * a.Dt[ia * N + ib], ww = sc * a.W[ia * N + ib];
+            gl += one_e_energy(la, lb, P, AB, a, wd, ww);
+            for (int i = 0; i < 3; ++i) {
+                int l2[3] = {la[0], la[1], la[2]};
+                l2[i] += 2;
+                gxa -= one_e_energy(l2, lb, P, AB, a, wd, ww);
+                l2[i] -= 1;
+                double g = 2.0 * alpha * one_e_energy(l2, lb, P, AB, a, wd, ww);
+                if (la[i] > 0) {
+                    l2[i] -= 2;
+                    g -= la[i] * one_e_energy(l2, lb, P, AB, a, wd, ww);
+                }
+                gAc[i] += g;
+                int m2[3] = {lb[0], lb[1], lb[2]};
+                m2[i] += 2;
+                gxb -= one_e_energy(la, m2, P, AB, a, wd, ww);
+                m2[i] -= 1;
+                g = 2.0 * beta * one_e_energy(la, m2, P, AB, a, wd, ww);
+                if (lb[i] > 0) {
+                    m2[i] -= 2;
+                    g -= lb[i] * one_e_energy(la, m2, P, AB, a, wd, ww);
+                }
+                gBc[i] += g;
+            }
+        }
+    const int pA = a.poff[sA], pB = a.poff[sB];
+    atomicAdd(a.g_xpn + pA + ka, gxa);
+    atomicAdd(a.g_xpn + pB + kb, gxb);
+    atomicAdd(a.g_lin + pA + ka, gl);
+    atomicAdd(a.g_lin + pB + kb, gl);
+    for (int i = 0; i < 3; ++i) {
+        atomicAdd(a.g_cen + 3 * sA + i, gAc[i]);
+        atomicAdd(a.g_cen + 3 * sB + i, gBc[i]);
+    }
 }
 
 template <int KIND>
@@ -432,6 +549,16 @@ PairClassDev pcdev(const PairClass& c)
 const EriLaunch k_jk64[QBX_N_CLASSES] = {QBX_FOR_EACH_CLASS(QBX_E64JK)};
 const EriLaunch k_jk32[QBX_N_CLASSES] = {QBX_FOR_EACH_CLASS(QBX_E32JK)};
 const EriLaunch k_dense[QBX_N_CLASSES] = {QBX_FOR_EACH_CLASS(QBX_EDENSE)};
+
+EriLaunch grad_launcher(int cls)
+{
+    const int* L = qbx_class_l[cls];
+#define QBX_GR(a, b, c, d) \
+    if (L[0] == a && L[1] == b && L[2] == c && L[3] == d) return &launch_eri_grad<a, b, c, d>;
+    QBX_GR(0, 0, 0, 0) QBX_GR(1, 0, 0, 0) QBX_GR(1, 0, 1, 0) QBX_GR(1, 1, 0, 0) QBX_GR(1, 1, 1, 0) QBX_GR(1, 1, 1, 1)
+#undef QBX_GR
+    return nullptr;
+}
 
 EriLaunch schwarz_launcher(int la, int lb)
 {
@@ -971,6 +1098,7 @@ int qbx_basis_create(qbx_ctx* ctx, int32_t n_prim, const double* prim_cen, const
                         q.Kp = kp;
                         q.pad = bb;
                         b->pp.push_back(q);
+                        b->ppk.push_back(u | (v << 16));
                     }
                 P.ppn.push_back(kept);
                 for (int t = 0; t < 3; ++t) P.AB.push_back(AB[t]);
@@ -995,6 +1123,10 @@ int qbx_basis_create(qbx_ctx* ctx, int32_t n_prim, const double* prim_cen, const
         }
         b->d_sh.upload(ds, st);
         b->d_pp.upload(b->pp, st);
+        b->d_ppk.upload(b->ppk, st);
+        std::vector<int> poff(ns);
+        for (int s = 0; s < ns; ++s) poff[s] = b->shells[s].poff;
+        b->d_poff.upload(poff, st);
         for (int c = 0; c < QBX_N_PAIR_CLASSES; ++c) {
             PairClass& P = b->pc[c];
             P.d_shA.upload(P.shA, st);
@@ -1243,6 +1375,122 @@ int qbx_set_density_screening(qbx_ctx* ctx, double thr)
         if (!ctx) throw QbxError(QBX_EINVAL, "null context");
         if (!(thr >= 0.0)) throw QbxError(QBX_EINVAL, "density screening threshold must be >= 0");
         ctx->density_thr = thr;
+    });
+}
+
+int qbx_basis_shells(const qbx_basis* b, int32_t* n_shell, int32_t* n_radial, int32_t* shell_L,
+                     int32_t* shell_nprim, int32_t* shell_poff, int32_t* shell_bf, double* shell_sc,
+                     double* shell_cen, double* radial_xpn, double* radial_coef)
+{
+    return guarded([&] {
+        if (!b) throw QbxError(QBX_EINVAL, "null basis");
+        const int ns = (int)b->shells.size();
+        if (n_shell) *n_shell = ns;
+        if (n_radial) *n_radial = (int32_t)b->pexp.size();
+        for (int s = 0; s < ns; ++s) {
+            const HostShell& H = b->shells[s];
+            if (shell_L) shell_L[s] = H.L;
+            if (shell_nprim) shell_nprim[s] = H.nprim;
+            if (shell_poff) shell_poff[s] = H.poff;
+            for (int c = 0; c < QBX_MAXC; ++c) {
+                if (shell_bf) shell_bf[QBX_MAXC * s + c] = H.bf[c];
+                if (shell_sc) shell_sc[QBX_MAXC * s + c] = H.sc[c];
+            }
+            for (int t = 0; t < 3; ++t)
+                if (shell_cen) shell_cen[3 * s + t] = H.c[t];
+        }
+        for (size_t k = 0; k < b->pexp.size(); ++k) {
+            if (radial_xpn) radial_xpn[k] = b->pexp[k];
+            if (radial_coef) radial_coef[k] = b->pcoef[k];
+        }
+    });
+}
+
+int qbx_rhf_gradient(qbx_ctx* ctx, qbx_basis* b, const double* Dt, const double* W, int32_t n_nuc,
+                     const double* Z, const double* R, double thr, double* g_xpn, double* g_lin, double* g_cen)
+{
+    return guarded([&] {
+        check_ctx_basis(ctx, b);
+        if (!Dt || !W || !g_xpn || !g_lin || !g_cen) throw QbxError(QBX_EINVAL, "null density or output pointer");
+        if (n_nuc <= 0 || !Z || !R) throw QbxError(QBX_EINVAL, "the gradient needs at least one nucleus");
+        if (!(thr >= 0.0)) throw QbxError(QBX_EINVAL, "Schwarz threshold must be >= 0");
+        if (b->maxL > 1) throw QbxError(QBX_EUNSUP, "the gradient kernels cover s and p shells");
+        for (const HostShell& H : b->shells)
+            if (H.nprim > QBX_GRAD_MAXP)
+                throw QbxError(QBX_EUNSUP, "the gradient kernels take at most " + std::to_string(QBX_GRAD_MAXP) +
+                                               " primitives per shell");
+        QBX_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        ensure_schwarz(b);
+        ensure_tasks(b, thr, QBX_PREC_FP64, 0.0);
+        const size_t NN = (size_t)b->nbf * b->nbf;
+        const int ns = (int)b->shells.size(), nr = (int)b->pexp.size();
+        DevBuf<double> dD, dW, dZ, dR, gx, gl, gc;
+        dD.upload(std::vector<double>(Dt, Dt + NN), st);
+        dW.upload(std::vector<double>(W, W + NN), st);
+        dZ.upload(std::vector<double>(Z, Z + n_nuc), st);
+        dR.upload(std::vector<double>(R, R + 3 * n_nuc), st);
+        gx.ensure(nr);
+        gl.ensure(nr);
+        gc.ensure(3 * (size_t)ns);
+        zero_async(gx.p, nr, st);
+        zero_async(gl.p, nr, st);
+        zero_async(gc.p, 3 * (size_t)ns, st);
+        // one-electron part
+        OneGradArgs o;
+        std::memset(&o, 0, sizeof(o));
+        o.sh = b->d_sh.p;
+        o.pp = b->d_pp.p;
+        o.ppk = b->d_ppk.p;
+        o.pA = b->d_p1A.p;
+        o.pB = b->d_p1B.p;
+        o.po = b->d_p1o.p;
+        o.pn = b->d_p1n.p;
+        o.npair = (int)b->p1.size();
+        o.N = b->nbf;
+        o.boys = ctx->boys.p;
+        o.bexp = ctx->bexp.p;
+        o.nnuc = n_nuc;
+        o.Z = dZ.p;
+        o.R = dR.p;
+        o.Dt = dD.p;
+        o.W = dW.p;
+        o.poff = b->d_poff.p;
+        o.g_xpn = gx.p;
+        o.g_lin = gl.p;
+        o.g_cen = gc.p;
+        const int64_t threads = (int64_t)o.npair * kGradPairPrims;
+        if (threads > 0) {
+            hipLaunchKernelGGL(one_e_grad_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, o);
+            QBX_HIP(hipGetLastError());
+        }
+        // two-electron part
+        for (int cls = 0; cls < QBX_N_CLASSES; ++cls) {
+            const TaskSet& t64 = b->t64[cls];
+            if (t64.nwave <= 0) continue;
+            EriLaunch f = grad_launcher(cls);
+            if (!f) throw QbxError(QBX_EINTERNAL, "no gradient kernel for an occupied class");
+            EriArgs a = base_args(b, cls);
+            const int lt = qbx_class_l[cls][0] + qbx_class_l[cls][1] + qbx_class_l[cls][2] + qbx_class_l[cls][3];
+            a.boysl = ctx->boysl.p + (size_t)(lt + 1) * QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE;
+            a.D = dD.p;
+            a.prefix = t64.prefix.p;
+            a.koff = t64.koff.p;
+            a.cpre = t64.cpre.p;
+            a.total = t64.total;
+            a.nwave = t64.nwave;
+            a.nbra = t64.nbra;
+            a.ppk = b->d_ppk.p;
+            a.shell_poff = b->d_poff.p;
+            a.g_xpn = gx.p;
+            a.g_lin = gl.p;
+            a.g_cen = gc.p;
+            f(a, st);
+        }
+        QBX_HIP(hipMemcpyAsync(g_xpn, gx.p, nr * sizeof(double), hipMemcpyDeviceToHost, st));
+        QBX_HIP(hipMemcpyAsync(g_lin, gl.p, nr * sizeof(double), hipMemcpyDeviceToHost, st));
+        QBX_HIP(hipMemcpyAsync(g_cen, gc.p, 3 * (size_t)ns * sizeof(double), hipMemcpyDeviceToHost, st));
+        QBX_HIP(hipStreamSynchronize(st));
     });
 }
 

@@ -27,6 +27,7 @@
 
 #define QBX_MAXL 2           // largest shell angular momentum in the kernels
 #define QBX_MAXC 6           // Cartesian components of the largest shell
+#define QBX_GRAD_MAXP 8      // radial primitives per shell in the gradient kernels
 
 struct DevShell {
     double x, y, z;

@@ -48,12 +48,24 @@ struct EriArgs {
     // EriClass::NE * NCD * 64 values of the kernel's arithmetic type
     void* ws;
     int ws_slots;
+    // gradient kernels: radial indices (k_a | k_b << 16) of every primitive pair,
+    // first radial primitive of every shell, and the outputs: per radial
+    // primitive dE/d(exponent) and c dE/dc, per shell dE/d(centre)
+    const int* ppk;
+    const int* shell_poff;
+    double* g_xpn;
+    double* g_lin;
+    double* g_cen;
 };
 
 
 // Defined (explicitly instantiated) in the generated per-class translation units.
 template <int LA, int LB, int LC, int LD, typename R, int MODE>
 void launch_eri(const EriArgs& a, hipStream_t s);
+
+// Gradient kernels (classes whose shells are all s or p), generated units eri_<class>_grad.hip.
+template <int LA, int LB, int LC, int LD>
+void launch_eri_grad(const EriArgs& a, hipStream_t s);
 
 typedef void (*EriLaunch)(const EriArgs&, hipStream_t);
 }  // namespace qbx

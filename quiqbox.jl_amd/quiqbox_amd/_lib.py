@@ -44,6 +44,15 @@ SIGNATURES = {
     "qbx_last_timing": (_i32, [_vp, _pf, _pf]),
     "qbx_set_mixed_cut": (_i32, [_vp, _d]),
     "qbx_set_density_screening": (_i32, [_vp, _d]),
+    "qbx_basis_shells": (_i32, [_vp, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd]),
+    "qbx_rhf_gradient": (_i32, [_vp, _vp, _pd, _pd, _i32, _pd, _pd, _d, _pd, _pd, _pd]),
+    "qbx_gemm_bf16": (_i32, [_i32, _i32, _i32, _i32, _i32, ctypes.c_float, _vp, _i32, _vp, _i32, ctypes.c_float,
+                             _vp, _i32, _i32, _vp, _vp, _vp]),
+    "qbx_mlp_elementwise": (_i32, [_i32, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "qbx_mlp_mse": (_i32, [_i32, _i32, _vp, _vp, _vp, _vp, ctypes.c_float, _vp, _vp, _vp, _vp]),
+    "qbx_mlp_colsum": (_i32, [_i32, _i32, _vp, _vp, _vp]),
+    "qbx_mlp_adam": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                            ctypes.c_float, _i32, _vp]),
     "qbx_last_error": (_i32, [ctypes.c_char_p, ctypes.c_size_t]),
 }
 

@@ -178,6 +178,40 @@ class GPUBasis:
                                                  ctypes.c_void_p(dDK_ptr), ctypes.c_void_p(dG_ptr), float(thr),
                                                  int(precision), ctypes.c_void_p(stream or 0)))
 
+    def shells(self):
+        """The kernels' shell table (qbx_basis_shells): per shell L, radial
+        primitive offset and count, component functions and scales, centre;
+        per radial primitive exponent and coefficient."""
+        L = _lib.load()
+        ns, nr = ctypes.c_int32(), ctypes.c_int32()
+        _lib.check(L.qbx_basis_shells(self.h, ctypes.byref(ns), ctypes.byref(nr), None, None, None, None, None, None,
+                                      None, None))
+        ns, nr = ns.value, nr.value
+        Ls, npr, poff = (np.zeros(ns, dtype=np.int32) for _ in range(3))
+        bf = np.zeros(6 * ns, dtype=np.int32)
+        sc = np.zeros(6 * ns)
+        cen = np.zeros(3 * ns)
+        xpn, coef = np.zeros(nr), np.zeros(nr)
+        _lib.check(L.qbx_basis_shells(self.h, None, None, _lib.ip(Ls), _lib.ip(npr), _lib.ip(poff), _lib.ip(bf),
+                                      _lib.dp(sc), _lib.dp(cen), _lib.dp(xpn), _lib.dp(coef)))
+        return dict(L=Ls, nprim=npr, poff=poff, bf=bf.reshape(ns, 6), sc=sc.reshape(ns, 6), center=cen.reshape(ns, 3),
+                    xpn=xpn, coef=coef)
+
+    def rhf_gradient(self, Dt, W, Z, R, thr=1e-12):
+        """qbx_rhf_gradient: per radial primitive dE/d(exponent) and c dE/dc,
+        per shell dE/d(centre), every function's weights held fixed."""
+        if self.T is not None:
+            raise ValueError("the gradient needs a standard (single-centre) basis")
+        Dt, W = _lib.f64(Dt), _lib.f64(W)
+        Z, R = _lib.f64(Z), _lib.f64(R).reshape(-1)
+        sh = self.shells()
+        nr, ns = len(sh["xpn"]), len(sh["L"])
+        gx, gl, gc = np.zeros(nr), np.zeros(nr), np.zeros(3 * ns)
+        with self.ctx.lock:
+            _lib.check(_lib.load().qbx_rhf_gradient(self.ctx.h, self.h, _lib.dp(Dt), _lib.dp(W), len(Z), _lib.dp(Z),
+                                                    _lib.dp(R), float(thr), _lib.dp(gx), _lib.dp(gl), _lib.dp(gc)))
+        return gx, gl, gc.reshape(ns, 3), sh
+
     def jk_stats(self, thr=1e-12, cut=0.0):
         n = _lib.load().qbx_n_classes()
         arrs = [np.zeros(n, dtype=np.int64) for _ in range(4)]

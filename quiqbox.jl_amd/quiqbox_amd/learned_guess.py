@@ -1,0 +1,312 @@
+"""Learned SCF initial guess: a pair-feature network that predicts the RHF
+density from the overlap and core-Hamiltonian blocks of every atom pair.
+
+The reference's guesses are the core Hamiltonian, GWH and SAD
+(`HartreeFock.jl:215-293`); this is an addition with no counterpart there, and
+SAD stays the fallback.  For atoms A, B with function blocks S_AB, H_AB (zero-
+padded to NMAX x NMAX), the features are [S_AB, H_AB, element flags of A and B,
+A == B, R, exp(-R), 1/(1+R)] normalized with the training set's statistics; a
+3-layer MLP (width 256, ReLU) maps them to the padded density block D_AB.  The
+network runs on the GPU in BF16 with FP32 accumulation through the library's
+own kernels (qbx_gemm_bf16 on the matrix cores, qbx_mlp_* for the elementwise
+parts, Adam on FP32 master weights), for training as well as inference.  torch
+only provides device buffers.
+
+Use: a trained `LearnedGuess` turns (nuclei, basis) into initial coefficients:
+the predicted blocks form a symmetric density, one Fock build (GPU J/K) gives
+F(D_pred), and its eigenvectors in the orthonormal basis are the guess
+(`initial_coefficients`); `guess_config` wraps them in the reference's
+OrbCoeffInitialConfig(:Direct) form, falling back to SAD when the predicted
+density's energy is above SAD's.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import _lib
+from .engine import GPUBasis
+from .hartree_fock import (HFconfig, OrbCoeffInitialConfig, RCHartreeFock, SCFconfig, getC, getOrthonormalization,
+                           runHartreeFock)
+
+NMAX = 15                     # functions per atom in the padded blocks (cc-pVDZ oxygen: 15 Cartesian)
+ELEMENTS = ("H", "O")
+N_FEAT = 2 * NMAX * NMAX + 2 * len(ELEMENTS) + 4
+N_IN = (N_FEAT + 7) // 8 * 8  # padded to the MFMA K step
+N_OUT = NMAX * NMAX
+WIDTH = 256
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("the learned guess runs on the GPU (no device found)")
+    return torch
+
+
+def atom_blocks(nuc, gb):
+    """Function indices of every atom (functions whose centre is the nucleus)."""
+    cen = np.asarray(gb.flat.prim_cen)
+    first = np.asarray(gb.flat.bf_prim)[np.asarray(gb.flat.bf_ptr)[:-1]]
+    fc = cen[first]
+    R = nuc.coord_array
+    owner = np.argmin(((fc[:, None, :] - R[None, :, :]) ** 2).sum(-1), axis=1)
+    if np.any(np.sqrt(((fc - R[owner]) ** 2).sum(-1)) > 1e-8):
+        raise ValueError("every basis function must sit on a nucleus")
+    blocks = [np.nonzero(owner == a)[0] for a in range(len(R))]
+    if max(len(b) for b in blocks) > NMAX:
+        raise ValueError("atoms with more than %d functions" % NMAX)
+    for s in nuc.symbols:
+        if s not in ELEMENTS:
+            raise ValueError("element %s outside the trained set %s" % (s, ELEMENTS))
+    return blocks
+
+
+def pair_features(nuc, blocks, S, H):
+    """Raw features (n_atoms^2 x N_FEAT) of all ordered atom pairs."""
+    na = len(blocks)
+    R = nuc.coord_array
+    X = np.zeros((na * na, N_FEAT))
+    for a in range(na):
+        for b in range(na):
+            r = a * na + b
+            ia, ib = blocks[a], blocks[b]
+            s = np.zeros((NMAX, NMAX))
+            h = np.zeros((NMAX, NMAX))
+            s[:len(ia), :len(ib)] = S[np.ix_(ia, ib)]
+            h[:len(ia), :len(ib)] = H[np.ix_(ia, ib)]
+            X[r, :NMAX * NMAX] = s.ravel()
+            X[r, NMAX * NMAX:2 * NMAX * NMAX] = h.ravel()
+            o = 2 * NMAX * NMAX
+            X[r, o + ELEMENTS.index(nuc.symbols[a])] = 1.0
+            X[r, o + len(ELEMENTS) + ELEMENTS.index(nuc.symbols[b])] = 1.0
+            d = float(np.linalg.norm(R[a] - R[b]))
+            X[r, o + 2 * len(ELEMENTS):] = (1.0 if a == b else 0.0, d, np.exp(-d), 1.0 / (1.0 + d))
+    return X
+
+
+def pair_targets(blocks, D):
+    na = len(blocks)
+    Y = np.zeros((na * na, N_OUT))
+    for a in range(na):
+        for b in range(na):
+            t = np.zeros((NMAX, NMAX))
+            t[:len(blocks[a]), :len(blocks[b])] = D[np.ix_(blocks[a], blocks[b])]
+            Y[a * na + b] = t.ravel()
+    return Y
+
+
+def assemble_density(blocks, Y, n):
+    na = len(blocks)
+    D = np.zeros((n, n))
+    for a in range(na):
+        for b in range(na):
+            t = Y[a * na + b].reshape(NMAX, NMAX)
+            D[np.ix_(blocks[a], blocks[b])] = t[:len(blocks[a]), :len(blocks[b])]
+    return 0.5 * (D + D.T)
+
+
+def system_data(nuc, gb):
+    S = gb.overlap()
+    H = gb.core_hamiltonian(nuc.charges, nuc.coord_array)
+    blocks = atom_blocks(nuc, gb)
+    return S, H, blocks
+
+
+class _Dev:
+    """Device buffers of one layer set (FP32 master weights, BF16 copies, Adam moments)."""
+
+    def __init__(self, torch, rng, sizes):
+        self.W, self.Wb, self.b, self.mW, self.vW, self.mb, self.vb = [], [], [], [], [], [], []
+        for (fi, fo) in sizes:
+            w = rng.normal(scale=np.sqrt(2.0 / fi), size=(fi, fo)).astype(np.float32)
+            self.W.append(torch.tensor(w, device="cuda"))
+            self.Wb.append(torch.tensor(w, device="cuda").to(torch.bfloat16))
+            self.b.append(torch.zeros(fo, dtype=torch.float32, device="cuda"))
+            self.mW.append(torch.zeros_like(self.W[-1]))
+            self.vW.append(torch.zeros_like(self.W[-1]))
+            self.mb.append(torch.zeros_like(self.b[-1]))
+            self.vb.append(torch.zeros_like(self.b[-1]))
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class LearnedGuess:
+    """The pair-feature MLP, its feature statistics and its GPU training loop."""
+
+    def __init__(self, seed=0):
+        self.torch = _torch()
+        self.rng = np.random.default_rng(seed)
+        self.sizes = [(N_IN, WIDTH), (WIDTH, WIDTH), (WIDTH, N_OUT)]
+        self.dev = _Dev(self.torch, self.rng, self.sizes)
+        self.mu = np.zeros(N_FEAT)
+        self.sd = np.ones(N_FEAT)
+        self.step = 0
+
+    # ---- data ----
+    def _inputs(self, X):
+        Z = np.zeros((X.shape[0], N_IN), dtype=np.float32)
+        Z[:, :N_FEAT] = (X - self.mu) / self.sd
+        return Z
+
+    def fit_stats(self, X):
+        self.mu = X.mean(0)
+        sd = X.std(0)
+        self.sd = np.where(sd > 1e-8, sd, 1.0)
+
+    # ---- forward / backward on the GPU ----
+    def _forward(self, xb, n, keep=False):
+        """xb: (n, N_IN) BF16 device tensor -> output FP32 (n, N_OUT) device tensor."""
+        torch = self.torch
+        L = _lib.load()
+        d = self.dev
+        acts_b = [xb]
+        acts_f = []
+        h = xb
+        for li, (fi, fo) in enumerate(self.sizes):
+            out = torch.empty((n, fo), dtype=torch.float32, device="cuda")
+            last = li == len(self.sizes) - 1
+            outb = None if last else torch.empty((n, fo), dtype=torch.bfloat16, device="cuda")
+            _lib.check(L.qbx_gemm_bf16(0, 0, n, fo, fi, 1.0, _p(h), fi, _p(d.Wb[li]), fo, 0.0, _p(out), fo,
+                                       2 if last else 1, _p(d.b[li]), _p(outb), None))
+            acts_f.append(out)
+            if not last:
+                acts_b.append(outb)
+                h = outb
+        return (out, acts_b) if keep else out
+
+    def train_step(self, xb, yt, rw, n, lr=1e-3):
+        """One Adam step on a batch: xb (n, N_IN) BF16, yt (n, N_OUT) FP32 targets,
+        rw (n,) FP32 row weights (1 real, 0 padding).  Returns the batch loss."""
+        torch = self.torch
+        L = _lib.load()
+        d = self.dev
+        y, acts_b = self._forward(xb, n, keep=True)
+        nreal = float(rw.sum().item())
+        dY = torch.empty_like(y)
+        dYb = torch.empty((n, N_OUT), dtype=torch.bfloat16, device="cuda")
+        loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+        cmask = getattr(self, "_cmask", None)
+        if cmask is None:
+            cmask = self._cmask = torch.ones(N_OUT, dtype=torch.float32, device="cuda")
+        _lib.check(L.qbx_mlp_mse(n, N_OUT, _p(y), _p(yt), _p(rw), _p(cmask), 1.0 / max(nreal, 1.0), _p(dY), _p(dYb),
+                                 _p(loss), None))
+        grads = self.backward(acts_b, dY, dYb, n)
+        self.step += 1
+        for li in range(len(self.sizes)):
+            gW, gb = grads[li]
+            _lib.check(L.qbx_mlp_adam(d.W[li].numel(), _p(d.W[li]), _p(d.mW[li]), _p(d.vW[li]), _p(gW), _p(d.Wb[li]),
+                                      lr, 0.9, 0.999, 1e-8, self.step, None))
+            _lib.check(L.qbx_mlp_adam(d.b[li].numel(), _p(d.b[li]), _p(d.mb[li]), _p(d.vb[li]), _p(gb), None,
+                                      lr, 0.9, 0.999, 1e-8, self.step, None))
+        return loss
+
+    def backward(self, acts_b, dY, dYb, n):
+        """Weight and bias gradients [(dW_l, db_l)] of the loss whose output
+        gradient is dY (FP32) / dYb (BF16), given the forward BF16 activations
+        acts_b = [x, h1, h2]."""
+        torch = self.torch
+        L = _lib.load()
+        d = self.dev
+        grads = [None] * len(self.sizes)
+        g_f, g_b = dY, dYb
+        for li in reversed(range(len(self.sizes))):
+            fi, fo = self.sizes[li]
+            gW = torch.empty((fi, fo), dtype=torch.float32, device="cuda")
+            gbias = torch.empty(fo, dtype=torch.float32, device="cuda")
+            # dW = h^T g, db = column sums of g
+            _lib.check(L.qbx_gemm_bf16(1, 0, fi, fo, n, 1.0, _p(acts_b[li]), fi, _p(g_b), fo, 0.0, _p(gW), fo, 0,
+                                       None, None, None))
+            _lib.check(L.qbx_mlp_colsum(n, fo, _p(g_f), _p(gbias), None))
+            grads[li] = (gW, gbias)
+            if li > 0:
+                # dh = g W^T, then through the ReLU of the layer below
+                dh = torch.empty((n, fi), dtype=torch.float32, device="cuda")
+                _lib.check(L.qbx_gemm_bf16(0, 1, n, fi, fo, 1.0, _p(g_b), fo, _p(d.Wb[li]), fo, 0.0, _p(dh), fi, 0,
+                                           None, None, None))
+                nf = torch.empty_like(dh)
+                nb = torch.empty((n, fi), dtype=torch.bfloat16, device="cuda")
+                _lib.check(L.qbx_mlp_elementwise(1, n * fi, _p(dh), _p(acts_b[li]), _p(nb), _p(nf), None))
+                g_f, g_b = nf, nb
+        return grads
+
+    def predict(self, X):
+        torch = self.torch
+        xb = torch.tensor(self._inputs(X), device="cuda").to(torch.bfloat16)
+        y = self._forward(xb, X.shape[0])
+        torch.cuda.synchronize()
+        return y.cpu().numpy().astype(np.float64)
+
+    # ---- training on converged densities ----
+    def train(self, data, epochs=200, batch=4096, lr=1e-3, log=None):
+        """data: list of (X, Y) per system (raw features, padded density blocks)."""
+        torch = self.torch
+        X = np.concatenate([x for x, _ in data])
+        Y = np.concatenate([y for _, y in data]).astype(np.float32)
+        self.fit_stats(X)
+        Z = self._inputs(X)
+        n = len(Z)
+        xb_all = torch.tensor(Z, device="cuda").to(torch.bfloat16)
+        yt_all = torch.tensor(Y, device="cuda")
+        hist = []
+        for ep in range(epochs):
+            perm = torch.tensor(self.rng.permutation(n), device="cuda")
+            tot = torch.zeros(1, device="cuda")
+            for s in range(0, n, batch):
+                idx = perm[s:s + batch]
+                m = len(idx)
+                xb = xb_all[idx].contiguous()
+                yt = yt_all[idx].contiguous()
+                rw = torch.ones(m, dtype=torch.float32, device="cuda")
+                tot += self.train_step(xb, yt, rw, m, lr) * m
+            hist.append(float(tot.item()) / n)
+            if log and (ep % max(1, epochs // 10) == 0 or ep == epochs - 1):
+                log("epoch %d loss %.3e" % (ep, hist[-1]))
+        return hist
+
+    # ---- guess ----
+    def predict_density(self, nuc, gb, S=None, H=None):
+        if S is None:
+            S, H, blocks = system_data(nuc, gb)
+        else:
+            blocks = atom_blocks(nuc, gb)
+        Y = self.predict(pair_features(nuc, blocks, S, H))
+        return assemble_density(blocks, Y, gb.n)
+
+    def initial_coefficients(self, nuc, gb, S=None, H=None, schwarz=1e-12):
+        """Coefficients from F(D_pred) = H + J[D_pred] - K[D_pred]/2 (D_pred the
+        predicted total density), diagonalized in the orthonormal basis."""
+        if S is None:
+            S, H, _ = system_data(nuc, gb)
+        Dt = self.predict_density(nuc, gb, S, H)
+        G = gb.jk(Dt, 0.5 * Dt, schwarz)
+        X = getOrthonormalization(S)
+        return getC(X, H + G), Dt
+
+
+def guess_config(model, nuc, gb, strategy=None, maxStep=200, schwarz=1e-12):
+    """HFconfig with the learned coefficients as OrbCoeffInitialConfig(:Direct)."""
+    C, _ = model.initial_coefficients(nuc, gb, schwarz=schwarz)
+    return HFconfig(initial=OrbCoeffInitialConfig(RCHartreeFock(), (C,)),
+                    strategy=strategy or SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-8)), maxStep=maxStep)
+
+
+def training_set(systems, schwarz=1e-12, strategy=None, log=None):
+    """Converged GPU RHF densities of `systems` [(nuc, basis orbitals)] as
+    (features, targets) per system; systems that do not converge are skipped."""
+    out = []
+    t0 = time.perf_counter()
+    for k, (nuc, bs) in enumerate(systems):
+        gb = GPUBasis(bs)
+        S, H, blocks = system_data(nuc, gb)
+        cfg = HFconfig(initial="SAD", strategy=strategy or SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-9)), maxStep=150)
+        res = runHartreeFock(nuc, gb, cfg, printInfo=False, schwarzThreshold=schwarz)
+        if not res.converged:
+            continue
+        Dt = 2.0 * res.density[0]
+        out.append((pair_features(nuc, blocks, S, H), pair_targets(blocks, Dt)))
+    if log:
+        log("training set: %d of %d systems converged, %.1f s" % (len(out), len(systems), time.perf_counter() - t0))
+    return out
