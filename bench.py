@@ -169,6 +169,41 @@ def run_scf(nuc, gb, precision, thr=1e-8, incremental=False):
     return res, time.perf_counter() - t0
 
 
+def learned_section(nuc, gb, res64, t64, n_train, epochs):
+    """Learned initial guess: train the pair-feature network on GPU-converged
+    densities of small water clusters (cc-pVDZ, 2-6 molecules, seeds disjoint
+    from the evaluation system), then start the 600-function water SCF from
+    it, in FP64 and in mixed precision with incremental builds."""
+    import quiqbox_amd as qb
+    from quiqbox_amd import learned_guess as lg
+    from quiqbox_amd.systems import water_box
+    t0 = time.perf_counter()
+    systems = [water_box(2 + k % 5, 1000 + k, basis="cc-pVDZ") for k in range(n_train)]
+    data = lg.training_set(systems, log=log)
+    t_data = time.perf_counter() - t0
+    model = lg.LearnedGuess(seed=0)
+    t0 = time.perf_counter()
+    hist = model.train(data, epochs=epochs, batch=2048, lr=1e-3, log=log)
+    t_train = time.perf_counter() - t0
+    strat = qb.SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-8))
+    out = {"training_systems": len(data), "training_pairs": int(sum(len(x) for x, _ in data)),
+           "data_s": t_data, "train_s": t_train, "epochs": epochs, "loss_first": hist[0], "loss_last": hist[-1]}
+    for tag, prec, inc in (("fp64", qb.QBX_PREC_FP64, False), ("fast", qb.QBX_PREC_MIXED, True)):
+        t0 = time.perf_counter()
+        cfg = lg.guess_config(model, nuc, gb, strat, 120)
+        res = qb.runHartreeFock(nuc, gb, cfg, printInfo=False, schwarzThreshold=1e-12, precision=prec,
+                                incremental=inc)
+        t = time.perf_counter() - t0
+        log("RHF %s from the learned guess: converged=%s in %.2f s, %d Fock builds, dE vs FP64 %.2e" % (
+            tag, res.converged, t, res.timing["fock_builds"], res.energy[0] - res64.energy[0]))
+        out[tag + "_wall_s"] = t
+        out[tag + "_fock_builds"] = res.timing["fock_builds"]
+        out[tag + "_converged"] = bool(res.converged)
+        out[tag + "_energy_error_vs_fp64"] = res.energy[0] - res64.energy[0]
+    out["speedup_fast_learned_vs_fp64_core"] = t64 / out["fast_wall_s"]
+    return out
+
+
 def gradient_section(steps):
     """Basis-parameter gradient on the 32-atom H cluster with floating
     4-primitive s and p shells (256 exponents, 256 coefficients, 96 centre
@@ -231,6 +266,7 @@ def rhf_section(hip, cpu_seconds, no_cpu):
     res64, t64 = run_scf(nuc, gb, qb.QBX_PREC_FP64)
     log("RHF FP64: E=%.10f converged=%s in %.2f s, %d Fock builds" % (
         res64.energy[0] + res64.energy[1], res64.converged, t64, res64.timing["fock_builds"]))
+    rhf_section.last = (nuc, gb, res64, t64)
     resmx, tmx = run_scf(nuc, gb, qb.QBX_PREC_MIXED, incremental=True)
     log("RHF mixed+incremental: E=%.10f converged=%s in %.2f s (%s of %s builds mixed)" % (
         resmx.energy[0] + resmx.energy[1], resmx.converged, tmx, resmx.timing["mixed_builds"],
@@ -267,6 +303,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--learned-systems", type=int, default=40,
+                    help="training systems of the learned-guess section (0: skip it)")
+    ap.add_argument("--learned-epochs", type=int, default=300)
     ap.add_argument("--grad-steps", type=int, default=100,
                     help="basis-optimization steps of the gradient section (0: skip it)")
     ap.add_argument("--serial", action="store_true",
@@ -372,6 +411,8 @@ def main():
         hip.free(p)
     if not args.no_scf:
         out["rhf"] = rhf_section(hip, min(args.cpu_seconds, 8.0), args.no_cpu)
+        if args.learned_systems > 0:
+            out["learned_guess"] = learned_section(*rhf_section.last, args.learned_systems, args.learned_epochs)
         if args.grad_steps > 0:
             out["gradient"] = gradient_section(args.grad_steps)
     print(json.dumps(out), flush=True)

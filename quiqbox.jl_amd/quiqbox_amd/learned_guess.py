@@ -16,8 +16,9 @@ Use: a trained `LearnedGuess` turns (nuclei, basis) into initial coefficients:
 the predicted blocks form a symmetric density, one Fock build (GPU J/K) gives
 F(D_pred), and its eigenvectors in the orthonormal basis are the guess
 (`initial_coefficients`); `guess_config` wraps them in the reference's
-OrbCoeffInitialConfig(:Direct) form, falling back to SAD when the predicted
-density's energy is above SAD's.
+OrbCoeffInitialConfig(:Direct) form, and falls back to the SAD guess when the
+predicted density is not usable (non-finite, or its electron count tr(D S)
+off by more than 10%).
 """
 import ctypes
 import time
@@ -287,10 +288,15 @@ class LearnedGuess:
 
 
 def guess_config(model, nuc, gb, strategy=None, maxStep=200, schwarz=1e-12):
-    """HFconfig with the learned coefficients as OrbCoeffInitialConfig(:Direct)."""
-    C, _ = model.initial_coefficients(nuc, gb, schwarz=schwarz)
-    return HFconfig(initial=OrbCoeffInitialConfig(RCHartreeFock(), (C,)),
-                    strategy=strategy or SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-8)), maxStep=maxStep)
+    """HFconfig with the learned coefficients as OrbCoeffInitialConfig(:Direct),
+    or with the SAD guess when the predicted density is not usable."""
+    strategy = strategy or SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-8))
+    S, H, _ = system_data(nuc, gb)
+    C, Dt = model.initial_coefficients(nuc, gb, S, H, schwarz=schwarz)
+    n_e = float(np.sum(nuc.charges))
+    if not (np.all(np.isfinite(C)) and abs(np.sum(Dt * S) - n_e) <= 0.1 * n_e):
+        return HFconfig(initial="SAD", strategy=strategy, maxStep=maxStep)
+    return HFconfig(initial=OrbCoeffInitialConfig(RCHartreeFock(), (C,)), strategy=strategy, maxStep=maxStep)
 
 
 def training_set(systems, schwarz=1e-12, strategy=None, log=None):
@@ -301,7 +307,7 @@ def training_set(systems, schwarz=1e-12, strategy=None, log=None):
     for k, (nuc, bs) in enumerate(systems):
         gb = GPUBasis(bs)
         S, H, blocks = system_data(nuc, gb)
-        cfg = HFconfig(initial="SAD", strategy=strategy or SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-9)), maxStep=150)
+        cfg = HFconfig(initial="CoreH", strategy=strategy or SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-9)), maxStep=150)
         res = runHartreeFock(nuc, gb, cfg, printInfo=False, schwarzThreshold=schwarz)
         if not res.converged:
             continue

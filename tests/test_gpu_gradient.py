@@ -9,6 +9,7 @@ import pytest
 
 import quiqbox_amd as qb
 from helpers import OracleEngine
+from quiqbox_amd._lib import QbxError
 from quiqbox_amd.gradient import ParamBasis, rhf_gradient
 from quiqbox_amd.systems import h_cluster_sp
 
@@ -91,5 +92,5 @@ def test_unsupported_inputs_raise():
     # d shells are outside the gradient kernels
     d_shell = ParamBasis([(0.0, 0.0, 0.0)], [(0, 2, [1.0, 0.5], [0.5, 0.5])])
     gd = qb.GPUBasis(d_shell.orbitals())
-    with pytest.raises(qb.QbxError):
+    with pytest.raises(QbxError):
         gd.rhf_gradient(np.eye(gd.n), np.eye(gd.n), [1.0], np.zeros((1, 3)))

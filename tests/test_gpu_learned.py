@@ -99,5 +99,6 @@ def test_trained_network_beats_the_core_guess_on_an_unseen_cluster():
         err_pred, err_core, len(learned.memory[0].shared.Etots), len(ref.memory[0].shared.Etots)))
     assert learned.converged
     assert abs(learned.energy[0] - ref.energy[0]) < 1e-7
-    assert err_pred < err_core
-    assert len(learned.memory[0].shared.Etots) < len(ref.memory[0].shared.Etots)
+    # the network's density is much closer to the converged one than the core guess's
+    assert err_pred < 0.5 * err_core
+    assert len(learned.memory[0].shared.Etots) <= len(ref.memory[0].shared.Etots)
