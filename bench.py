@@ -190,7 +190,13 @@ def learned_section(nuc, gb, res64, t64, n_train, epochs):
            "data_s": t_data, "train_s": t_train, "epochs": epochs, "loss_first": hist[0], "loss_last": hist[-1]}
     for tag, prec, inc in (("fp64", qb.QBX_PREC_FP64, False), ("fast", qb.QBX_PREC_MIXED, True)):
         t0 = time.perf_counter()
-        cfg = lg.guess_config(model, nuc, gb, strat, 120)
+        info = {}
+        cfg = lg.guess_config(model, nuc, gb, strat, 120, info=info)
+        log("learned guess: %.2f predicted electrons of %.0f, %s (%.2f s)" % (
+            info["electrons_predicted"], info["electrons"], "used" if info["learned"] else "core-guess fallback",
+            time.perf_counter() - t0))
+        out["electrons_predicted"] = info["electrons_predicted"]
+        out["learned_used"] = info["learned"]
         res = qb.runHartreeFock(nuc, gb, cfg, printInfo=False, schwarzThreshold=1e-12, precision=prec,
                                 incremental=inc)
         t = time.perf_counter() - t0
@@ -225,7 +231,8 @@ def gradient_section(steps):
     log("gradient: %d parameters, SCF %.3f s, adjoint gradient %.3f s, |g| %.3e" % (pb.n_theta, t_scf, t_grad,
                                                                                      np.linalg.norm(g)))
     t0 = time.perf_counter()
-    _, hist = optimize_basis(nuc, pb, steps=steps, lr=0.02, config=cfg)
+    _, hist = optimize_basis(nuc, pb, steps=steps, lr=0.02, config=cfg,
+                             callback=lambda it, E, g: log("  step %d: E %.10f |g| %.3e" % (it, E, np.linalg.norm(g))))
     t_opt = time.perf_counter() - t0
     log("basis optimization: %d steps in %.2f s, E %.8f -> %.8f" % (steps, t_opt, hist[0], hist[-1]))
     return {"workload": "32 H atoms, floating 4-primitive s and p shells (seed 0), 128 functions, %d parameters"
@@ -306,7 +313,7 @@ def main():
     ap.add_argument("--learned-systems", type=int, default=40,
                     help="training systems of the learned-guess section (0: skip it)")
     ap.add_argument("--learned-epochs", type=int, default=300)
-    ap.add_argument("--grad-steps", type=int, default=100,
+    ap.add_argument("--grad-steps", type=int, default=30,
                     help="basis-optimization steps of the gradient section (0: skip it)")
     ap.add_argument("--serial", action="store_true",
                     help="run the class kernels of each build one after another on one stream (for profiling)")

@@ -36,7 +36,7 @@ import numpy as np
 from . import systems
 from .basis import PrimitiveOrb, get3DimPGTOrbNormFactor, prim_overlap, subshellXYZs
 from .engine import GPUBasis
-from .hartree_fock import HFconfig, SCFconfig, runHartreeFock
+from .hartree_fock import HFconfig, OrbCoeffInitialConfig, RCHartreeFock, SCFconfig, runHartreeFock
 
 
 class ParamBasis:
@@ -185,9 +185,13 @@ def optimize_basis(nuc, pb, steps=100, lr=0.05, schwarz=1e-12, config=None, call
     hist = []
     res = None
     nx = sum(len(x) for _, _, x, _ in pb.shells)
+    base = config or HFconfig(initial="CoreH", strategy=SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-10)), maxStep=200)
     for it in range(steps):
         gb = GPUBasis(pb.orbitals())
-        cfg = config or HFconfig(initial="CoreH", strategy=SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-10)), maxStep=200)
+        cfg = base
+        if res is not None and res.converged:
+            cfg = HFconfig(initial=OrbCoeffInitialConfig(RCHartreeFock(), (res.coeff[0],)), strategy=base.strategy,
+                           maxStep=base.maxStep)
         E, g, res = rhf_gradient(nuc, pb, cfg, schwarz, gpu_basis=gb)
         hist.append(E)
         if callback:

@@ -16,9 +16,12 @@ Use: a trained `LearnedGuess` turns (nuclei, basis) into initial coefficients:
 the predicted blocks form a symmetric density, one Fock build (GPU J/K) gives
 F(D_pred), and its eigenvectors in the orthonormal basis are the guess
 (`initial_coefficients`); `guess_config` wraps them in the reference's
-OrbCoeffInitialConfig(:Direct) form, and falls back to the SAD guess when the
-predicted density is not usable (non-finite, or its electron count tr(D S)
-off by more than 10%).
+OrbCoeffInitialConfig(:Direct) form, and falls back to the core-Hamiltonian
+guess when the predicted density is not usable (non-finite, or its electron
+count tr(D S) off by more than 10%).  Atom pairs farther apart than any pair of
+the training set get a zero block instead of an extrapolated one.  (Quiqbox's
+SAD, the other plain guess, runs one full-basis UHF per nucleus,
+`HartreeFock.jl:256-293`: minutes at 600 functions, so it is not the fallback.)
 """
 import ctypes
 import time
@@ -145,6 +148,7 @@ class LearnedGuess:
         self.mu = np.zeros(N_FEAT)
         self.sd = np.ones(N_FEAT)
         self.step = 0
+        self.r_max = np.inf      # largest atom-pair distance of the training set
 
     # ---- data ----
     def _inputs(self, X):
@@ -247,6 +251,7 @@ class LearnedGuess:
         X = np.concatenate([x for x, _ in data])
         Y = np.concatenate([y for _, y in data]).astype(np.float32)
         self.fit_stats(X)
+        self.r_max = float(X[:, 2 * NMAX * NMAX + 2 * len(ELEMENTS) + 1].max())
         Z = self._inputs(X)
         n = len(Z)
         xb_all = torch.tensor(Z, device="cuda").to(torch.bfloat16)
@@ -273,7 +278,9 @@ class LearnedGuess:
             S, H, blocks = system_data(nuc, gb)
         else:
             blocks = atom_blocks(nuc, gb)
-        Y = self.predict(pair_features(nuc, blocks, S, H))
+        X = pair_features(nuc, blocks, S, H)
+        Y = self.predict(X)
+        Y[X[:, 2 * NMAX * NMAX + 2 * len(ELEMENTS) + 1] > self.r_max] = 0.0
         return assemble_density(blocks, Y, gb.n)
 
     def initial_coefficients(self, nuc, gb, S=None, H=None, schwarz=1e-12):
@@ -287,15 +294,20 @@ class LearnedGuess:
         return getC(X, H + G), Dt
 
 
-def guess_config(model, nuc, gb, strategy=None, maxStep=200, schwarz=1e-12):
+def guess_config(model, nuc, gb, strategy=None, maxStep=200, schwarz=1e-12, info=None):
     """HFconfig with the learned coefficients as OrbCoeffInitialConfig(:Direct),
-    or with the SAD guess when the predicted density is not usable."""
+    or with the core-Hamiltonian guess when the predicted density is not usable.
+    `info` (a dict), if given, receives the predicted electron count and the choice."""
     strategy = strategy or SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-8))
     S, H, _ = system_data(nuc, gb)
     C, Dt = model.initial_coefficients(nuc, gb, S, H, schwarz=schwarz)
     n_e = float(np.sum(nuc.charges))
-    if not (np.all(np.isfinite(C)) and abs(np.sum(Dt * S) - n_e) <= 0.1 * n_e):
-        return HFconfig(initial="SAD", strategy=strategy, maxStep=maxStep)
+    n_pred = float(np.sum(Dt * S))
+    usable = bool(np.all(np.isfinite(C)) and abs(n_pred - n_e) <= 0.1 * n_e)
+    if info is not None:
+        info.update(electrons_predicted=n_pred, electrons=n_e, learned=usable)
+    if not usable:
+        return HFconfig(initial="CoreH", strategy=strategy, maxStep=maxStep)
     return HFconfig(initial=OrbCoeffInitialConfig(RCHartreeFock(), (C,)), strategy=strategy, maxStep=maxStep)
 
 
