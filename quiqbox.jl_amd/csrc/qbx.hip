@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -96,6 +97,8 @@ struct qbx_ctx {
     float last_total = 0.f;
     double mixed_cut = 1e-5;
     double density_thr = 0.0;      // > 0: density-screened J/K
+    bool xcd_local = false;        // per-XCD J/K copies (set by the probe at init)
+    int n_xcd = 1;
     std::mutex mu;
 };
 
@@ -492,16 +495,28 @@ __global__ void fill_zero_kernel(double* p, size_t n)
 }
 
 // G[s] = (J + J^T)/2 - (K + K^T)/2
-__global__ void symmetrize_kernel(const double* J, const double* K, double* G, int N, int nset)
+// G = (J + J^T)/2 - (K + K^T)/2, J and K summed over their `copies` copies
+// (stride NN * nset doubles).
+__global__ void symmetrize_kernel(const double* J, const double* K, double* G, int N, int nset, int copies)
 {
     const size_t NN = (size_t)N * N;
     size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= NN * nset) return;
     const size_t s = g / NN, r = g % NN;
     const size_t i = r / N, j = r % N;
-    const double* Js = J + s * NN;
-    const double* Ks = K + s * NN;
-    G[g] = 0.5 * (Js[i * N + j] + Js[j * N + i]) - 0.5 * (Ks[i * N + j] + Ks[j * N + i]);
+    double v = 0.0;
+    for (int c = 0; c < copies; ++c) {
+        const double* Js = J + (size_t)c * NN * nset + s * NN;
+        const double* Ks = K + (size_t)c * NN * nset + s * NN;
+        v += 0.5 * (Js[i * N + j] + Js[j * N + i]) - 0.5 * (Ks[i * N + j] + Ks[j * N + i]);
+    }
+    G[g] = v;
+}
+
+// Which XCD runs each block (one write per block).
+__global__ void xcc_probe_kernel(int* out)
+{
+    if (threadIdx.x == 0) out[blockIdx.x] = qbx_xcc_id();
 }
 
 // ---------------------------------------------------------------------------
@@ -807,10 +822,11 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
     ensure_schwarz(b);
     ensure_tasks(b, thr, mode, ctx->mixed_cut);
     const size_t NN = (size_t)b->nbf * b->nbf;
-    b->d_J.ensure(NN * nset);
-    b->d_K.ensure(NN * nset);
-    zero_async(b->d_J.p, NN * nset, st);
-    zero_async(b->d_K.p, NN * nset, st);
+    const int copies = ctx->xcd_local ? QBX_MAX_XCD : 1;
+    b->d_J.ensure(NN * nset * copies);
+    b->d_K.ensure(NN * nset * copies);
+    zero_async(b->d_J.p, NN * nset * copies, st);
+    zero_async(b->d_K.p, NN * nset * copies, st);
     std::vector<hipEvent_t> ev;
     if (ctx->timing) {
         ev.resize(2 * QBX_N_CLASSES + 2);
@@ -850,6 +866,8 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
         a.D = dD;
         a.J = b->d_J.p;
         a.K = b->d_K.p;
+        a.xcd_local = ctx->xcd_local ? 1 : 0;
+        a.jk_stride = NN * nset;
         a.nset = nset;
         const TaskSet& t64 = b->t64[cls];
         a.prefix = t64.prefix.p;
@@ -880,7 +898,7 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
     }
     const size_t tot = NN * nset;
     hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                       b->d_J.p, b->d_K.p, dG, b->nbf, nset);
+                       b->d_J.p, b->d_K.p, dG, b->nbf, nset, copies);
     QBX_HIP(hipGetLastError());
     if (ctx->timing) {
         QBX_HIP(hipEventRecord(ev[2 * QBX_N_CLASSES + 1], st));
@@ -935,6 +953,31 @@ int qbx_init(int32_t device, qbx_ctx** out)
         c->boys.upload(tab, c->stream);
         c->bexp.upload(et, c->stream);
         c->boysl.upload(rows, c->stream);
+        // Per-XCD J/K accumulation needs the XCC_ID register to name each
+        // block's XCD: check that it reads ids below QBX_MAX_XCD and that
+        // several XCDs show up (a single-XCD device keeps one copy).
+        {
+            const int nblk = 4096;
+            DevBuf<int> ids;
+            ids.ensure(nblk);
+            hipLaunchKernelGGL(xcc_probe_kernel, dim3(nblk), dim3(64), 0, c->stream, ids.p);
+            QBX_HIP(hipGetLastError());
+            std::vector<int> h(nblk);
+            QBX_HIP(hipMemcpyAsync(h.data(), ids.p, nblk * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            QBX_HIP(hipStreamSynchronize(c->stream));
+            bool seen[QBX_MAX_XCD] = {};
+            bool ok = true;
+            for (int v : h) {
+                if (v < 0 || v >= QBX_MAX_XCD) ok = false;
+                else seen[v] = true;
+            }
+            int n = 0;
+            for (bool x : seen) n += x ? 1 : 0;
+            c->n_xcd = n;
+            const char* env = std::getenv("QBX_XCD_LOCAL");
+            const bool off = env && env[0] == '0';
+            c->xcd_local = ok && n >= 2 && !off;
+        }
         QBX_HIP(hipStreamSynchronize(c->stream));
         *out = c.release();
     });

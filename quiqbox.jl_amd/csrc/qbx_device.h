@@ -45,6 +45,12 @@ struct PrimPair {
 
 __host__ __device__ constexpr int qbx_ncart(int L) { return (L + 1) * (L + 2) / 2; }
 
+#define QBX_MAX_XCD 8        // J/K copies of the per-XCD accumulation
+
+// XCD of the calling wave: hardware register XCC_ID (gfx940 and later),
+// s_getreg_b32 hwreg(HW_REG_XCC_ID = 20, offset 0, size 4).
+__device__ __forceinline__ int qbx_xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & (QBX_MAX_XCD - 1); }
+
 template <int L>
 __device__ __forceinline__ void qbx_boys(double T, double* __restrict__ F,
                                          const double* __restrict__ tab,

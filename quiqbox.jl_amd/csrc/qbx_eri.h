@@ -33,6 +33,27 @@ namespace qbx {
 template <typename T> using cptr = const __attribute__((address_space(4))) T*;
 template <typename T> __device__ __forceinline__ cptr<T> cst(const T* p) { return (cptr<T>)p; }
 
+// J/K accumulation.  With per-XCD copies every writer of a copy runs on one
+// XCD, so the add only has to be atomic within that XCD's L2: workgroup scope
+// keeps it there instead of sending each 8-byte add out to memory.
+struct JKSink {
+    double* J;
+    double* K;
+    bool local;
+    __device__ __forceinline__ JKSink(const EriArgs& a)
+    {
+        local = a.xcd_local != 0;
+        const size_t off = local ? (size_t)qbx_xcc_id() * a.jk_stride : 0;
+        J = a.J + off;
+        K = a.K + off;
+    }
+    __device__ __forceinline__ void add(double* p, double v) const
+    {
+        if (local) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else atomicAdd(p, v);
+    }
+};
+
 // Threads per block: classes with few accumulators (and so few registers) run
 // 8 or 16 waves per block, so that one LDS copy of the Boys rows (48 KB, three
 // copies fit a CU) serves more waves.
@@ -292,11 +313,12 @@ __global__ __launch_bounds__(eri_tpb(EriClass<LA, LB, LC, LD, R>::NACC)) void er
 #pragma unroll
     for (int n = 0; n < ND; ++n) fd[n] = fd[n] < 0 ? 0 : fd[n];
 
+    const JKSink sink(a);
     for (int s = 0; s < a.nset; ++s) {
         const double* __restrict__ DJ = a.D + (size_t)s * N * N;
         const double* __restrict__ DK = a.D + (size_t)(a.nset + s) * N * N;
-        double* Js = a.J + (size_t)s * N * N;
-        double* Ks = a.K + (size_t)s * N * N;
+        double* Js = sink.J + (size_t)s * N * N;
+        double* Ks = sink.K + (size_t)s * N * N;
         // blocks that every slice reads
         double dcd[NCD], dbc[NB * NC], dbd[NB * ND];
 #pragma unroll
@@ -348,28 +370,28 @@ __global__ __launch_bounds__(eri_tpb(EriClass<LA, LB, LC, LD, R>::NACC)) void er
                     }
                 // the bra is wave-uniform: one atomic per wave for J_ab
                 jab = wave_sum(fj * jab);
-                if (lane == 0) atomicAdd(Js + fa[ia] * N + fb[ib], jab);
+                if (lane == 0) sink.add(Js + fa[ia] * N + fb[ib], jab);
             }
             if (live) {
 #pragma unroll
-                for (int id = 0; id < ND; ++id) atomicAdd(Ks + fa[ia] * N + fd[id], fk * kad[id]);
+                for (int id = 0; id < ND; ++id) sink.add(Ks + fa[ia] * N + fd[id], fk * kad[id]);
 #pragma unroll
-                for (int ic = 0; ic < NC; ++ic) atomicAdd(Ks + fa[ia] * N + fc[ic], fk * kac[ic]);
+                for (int ic = 0; ic < NC; ++ic) sink.add(Ks + fa[ia] * N + fc[ic], fk * kac[ic]);
             }
         }
         if (live) {
 #pragma unroll
             for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
-                for (int id = 0; id < ND; ++id) atomicAdd(Js + fc[ic] * N + fd[id], fj * jcd[ic * ND + id]);
+                for (int id = 0; id < ND; ++id) sink.add(Js + fc[ic] * N + fd[id], fj * jcd[ic * ND + id]);
 #pragma unroll
             for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
-                for (int id = 0; id < ND; ++id) atomicAdd(Ks + fb[ib] * N + fd[id], fk * kbd[ib * ND + id]);
+                for (int id = 0; id < ND; ++id) sink.add(Ks + fb[ib] * N + fd[id], fk * kbd[ib * ND + id]);
 #pragma unroll
             for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
-                for (int ic = 0; ic < NC; ++ic) atomicAdd(Ks + fb[ib] * N + fc[ic], fk * kbc[ib * NC + ic]);
+                for (int ic = 0; ic < NC; ++ic) sink.add(Ks + fb[ib] * N + fc[ic], fk * kbc[ib * NC + ic]);
         }
     }
 }
@@ -546,11 +568,12 @@ void eri_staged_kernel(EriArgs a)
         for (int n = 0; n < NC; ++n) fc[n] = fc[n] < 0 ? 0 : fc[n];
 #pragma unroll
         for (int n = 0; n < ND; ++n) fd[n] = fd[n] < 0 ? 0 : fd[n];
+        const JKSink sink(a);
         for (int s = 0; s < a.nset; ++s) {
             const double* __restrict__ DJ = a.D + (size_t)s * N * N;
             const double* __restrict__ DK = a.D + (size_t)(a.nset + s) * N * N;
-            double* Js = a.J + (size_t)s * N * N;
-            double* Ks = a.K + (size_t)s * N * N;
+            double* Js = sink.J + (size_t)s * N * N;
+            double* Ks = sink.K + (size_t)s * N * N;
 #pragma unroll 1
             for (int iab = 0; iab < NA * NB; ++iab) {
                 const int ia = iab / NB, ib = iab - ia * NB;
@@ -573,12 +596,12 @@ void eri_staged_kernel(EriArgs a)
                         kad[id] += v * DK[fb_ * N + fc[ic]];
                     }
                 jab = wave_sum(fj * jab);
-                if (lane == 0) atomicAdd(Js + fa_ * N + fb_, jab);
+                if (lane == 0) sink.add(Js + fa_ * N + fb_, jab);
                 if (live) {
 #pragma unroll
-                    for (int ic = 0; ic < NC; ++ic) atomicAdd(Ks + fa_ * N + fc[ic], fk * kac[ic]);
+                    for (int ic = 0; ic < NC; ++ic) sink.add(Ks + fa_ * N + fc[ic], fk * kac[ic]);
 #pragma unroll
-                    for (int id = 0; id < ND; ++id) atomicAdd(Ks + fa_ * N + fd[id], fk * kad[id]);
+                    for (int id = 0; id < ND; ++id) sink.add(Ks + fa_ * N + fd[id], fk * kad[id]);
                 }
             }
             double jcd[NCD];
@@ -612,16 +635,16 @@ void eri_staged_kernel(EriArgs a)
                 }
                 if (live) {
 #pragma unroll
-                    for (int id = 0; id < ND; ++id) atomicAdd(Ks + fb_ * N + fd[id], fk * kbd[id]);
+                    for (int id = 0; id < ND; ++id) sink.add(Ks + fb_ * N + fd[id], fk * kbd[id]);
 #pragma unroll
-                    for (int ic = 0; ic < NC; ++ic) atomicAdd(Ks + fb_ * N + fc[ic], fk * kbc[ic]);
+                    for (int ic = 0; ic < NC; ++ic) sink.add(Ks + fb_ * N + fc[ic], fk * kbc[ic]);
                 }
             }
             if (live) {
 #pragma unroll
                 for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
-                    for (int id = 0; id < ND; ++id) atomicAdd(Js + fc[ic] * N + fd[id], fj * jcd[ic * ND + id]);
+                    for (int id = 0; id < ND; ++id) sink.add(Js + fc[ic] * N + fd[id], fj * jcd[ic * ND + id]);
             }
         }
     }

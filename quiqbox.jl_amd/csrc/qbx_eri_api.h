@@ -32,8 +32,13 @@ struct EriArgs {
     int nbra;
     int same;
     const double* D;   // [2*nset][N][N]: DJ sets then DK sets
-    double* J;         // [nset][N][N]
+    double* J;         // [copies][nset][N][N]
     double* K;
+    // xcd_local: each XCD accumulates into its own copy of J and K (copy =
+    // the XCC_ID of the wave, stride jk_stride doubles) with workgroup-scope
+    // atomics, which the XCD's L2 resolves; otherwise one copy, device scope
+    int xcd_local;
+    size_t jk_stride;
     int nset;
     int N;
     double* eri;       // dense mode, N^4

@@ -2,11 +2,15 @@
 density from the overlap and core-Hamiltonian blocks of every atom pair.
 
 The reference's guesses are the core Hamiltonian, GWH and SAD
-(`HartreeFock.jl:215-293`); this is an addition with no counterpart there, and
-SAD stays the fallback.  For atoms A, B with function blocks S_AB, H_AB (zero-
-padded to NMAX x NMAX), the features are [S_AB, H_AB, element flags of A and B,
-A == B, R, exp(-R), 1/(1+R)] normalized with the training set's statistics; a
-3-layer MLP (width 256, ReLU) maps them to the padded density block D_AB.  The
+(`HartreeFock.jl:215-293`); this is an addition with no counterpart there.
+For atoms A, B with function blocks S_AB and H_AB (zero-padded to NMAX x NMAX)
+the features are [S_AB, H_AB, element flags of A and B, A == B, R, exp(-R),
+1/(1+R)] normalized with the training set's statistics, where H_AB is the pair's
+local core Hamiltonian T_AB + V(A)_AB + V(B)_AB: the kinetic block plus the
+attraction of the two atoms' own nuclei only (the full core Hamiltonian holds the
+attraction of every nucleus, so its blocks would not mean the same in a small
+training cluster and a large one).  A 3-layer MLP (width 256, ReLU) maps the
+features to the padded density block D_AB.  The
 network runs on the GPU in BF16 with FP32 accumulation through the library's
 own kernels (qbx_gemm_bf16 on the matrix cores, qbx_mlp_* for the elementwise
 parts, Adam on FP32 master weights), for training as well as inference.  torch
@@ -67,7 +71,9 @@ def atom_blocks(nuc, gb):
 
 
 def pair_features(nuc, blocks, S, H):
-    """Raw features (n_atoms^2 x N_FEAT) of all ordered atom pairs."""
+    """Raw features (n_atoms^2 x N_FEAT) of all ordered atom pairs.  H is
+    (T, Vn): the kinetic matrix and the attraction matrix of every nucleus."""
+    T, Vn = H
     na = len(blocks)
     R = nuc.coord_array
     X = np.zeros((na * na, N_FEAT))
@@ -78,7 +84,10 @@ def pair_features(nuc, blocks, S, H):
             s = np.zeros((NMAX, NMAX))
             h = np.zeros((NMAX, NMAX))
             s[:len(ia), :len(ib)] = S[np.ix_(ia, ib)]
-            h[:len(ia), :len(ib)] = H[np.ix_(ia, ib)]
+            hl = T[np.ix_(ia, ib)] + Vn[a][np.ix_(ia, ib)]
+            if b != a:
+                hl = hl + Vn[b][np.ix_(ia, ib)]
+            h[:len(ia), :len(ib)] = hl
             X[r, :NMAX * NMAX] = s.ravel()
             X[r, NMAX * NMAX:2 * NMAX * NMAX] = h.ravel()
             o = 2 * NMAX * NMAX
@@ -111,10 +120,14 @@ def assemble_density(blocks, Y, n):
 
 
 def system_data(nuc, gb):
+    """S, (T, per-nucleus attraction matrices), atom blocks; one-electron
+    integrals from the GPU kernels."""
     S = gb.overlap()
-    H = gb.core_hamiltonian(nuc.charges, nuc.coord_array)
+    T = gb.kinetic()
+    Z, R = nuc.charges, nuc.coord_array
+    Vn = [gb.nuclear(Z[k:k + 1], R[k:k + 1]) for k in range(len(Z))]
     blocks = atom_blocks(nuc, gb)
-    return S, H, blocks
+    return S, (T, Vn), blocks
 
 
 class _Dev:
@@ -284,14 +297,16 @@ class LearnedGuess:
         return assemble_density(blocks, Y, gb.n)
 
     def initial_coefficients(self, nuc, gb, S=None, H=None, schwarz=1e-12):
-        """Coefficients from F(D_pred) = H + J[D_pred] - K[D_pred]/2 (D_pred the
-        predicted total density), diagonalized in the orthonormal basis."""
+        """Coefficients from F(D_pred) = Hcore + J[D_pred] - K[D_pred]/2 (D_pred
+        the predicted total density), diagonalized in the orthonormal basis."""
         if S is None:
             S, H, _ = system_data(nuc, gb)
         Dt = self.predict_density(nuc, gb, S, H)
+        T, Vn = H
+        Hcore = T + sum(Vn)
         G = gb.jk(Dt, 0.5 * Dt, schwarz)
         X = getOrthonormalization(S)
-        return getC(X, H + G), Dt
+        return getC(X, Hcore + G), Dt
 
 
 def guess_config(model, nuc, gb, strategy=None, maxStep=200, schwarz=1e-12, info=None):

@@ -86,6 +86,7 @@ def test_trained_network_beats_the_core_guess_on_an_unseen_cluster():
     assert ref.converged
     D_conv = 2.0 * ref.density[0]
     D_pred = model.predict_density(nuc, gb)
+    assert abs(np.sum(D_pred * gb.overlap()) - 50.0) < 5.0
     from quiqbox_amd.hartree_fock import getC
     S = gb.overlap()
     H = gb.core_hamiltonian(nuc.charges, nuc.coord_array)
