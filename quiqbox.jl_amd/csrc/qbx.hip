@@ -99,6 +99,7 @@ struct qbx_ctx {
     double density_thr = 0.0;      // > 0: density-screened J/K
     bool xcd_local = false;        // per-XCD J/K copies (set by the probe at init)
     int n_xcd = 1;
+    int lds_max = 65536;           // LDS bytes per block on this device
     std::mutex mu;
 };
 
@@ -116,6 +117,8 @@ struct TaskSet {
     DevBuf<int64_t> prefix;
     DevBuf<int> koff;
     DevBuf<int> cpre;          // waves (chunks of 64 kets) per bra, prefix sums
+    DevBuf<int> spre;          // K-row blocks (ROW_CHUNKS chunks) per bra, prefix sums
+    int nsuper = 0;
     int64_t total = 0;
     int nwave = 0;
     int nbra = 0;
@@ -162,36 +165,47 @@ namespace {
 //   F_m(T) = e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1))
 // at the top order, then the stable downward recurrence.
 // ---------------------------------------------------------------------------
+// F_0..F_mtop at T by the convergent series at the top order and the stable
+// downward recurrence, in long double; returns e^{-T}.
+long double boys_seq_ld(long double T, int mtop, long double* F)
+{
+    const long double e = expl(-T);
+    long double term = 1.0L / (2 * mtop + 1), sum = term;
+    for (int k = 1; k < 20000; ++k) {
+        term *= 2.0L * T / (2 * mtop + 2 * k + 1);
+        sum += term;
+        if (term < sum * 1e-21L) break;
+    }
+    F[mtop] = e * sum;
+    for (int m = mtop - 1; m >= 0; --m) F[m] = (2.0L * T * F[m + 1] + e) / (2 * m + 1);
+    return e;
+}
+
 void build_boys_table(std::vector<double>& tab, std::vector<double>& et, std::vector<double>& rows)
 {
-    static_assert(QBX_BOYSL_LMAX + QBX_BOYS_TAYLOR <= QBX_BOYS_NCOL, "Boys table too narrow");
-    static_assert(QBX_BOYSL_ROWS == QBX_BOYS_ROWS, "Boys row counts differ");
+    static_assert(QBX_BOYSL_LMAX + QBX_BOYSL_TAYLOR <= 24, "Boys row orders too high");
+    static_assert((QBX_BOYSL_ROWS - 2) * QBX_BOYSL_DT >= QBX_BOYS_TMAX, "Boys rows do not cover the table range");
     tab.assign((size_t)QBX_BOYS_ROWS * QBX_BOYS_NCOL, 0.0);
     et.assign(QBX_BOYS_ROWS, 0.0);
     rows.assign((size_t)(QBX_BOYSL_LMAX + 1) * QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE, 0.0);
-    const int mtop = QBX_BOYS_NCOL - 1;
+    long double F[24];
+    // one-electron kernels: F_0..F_16 on the 0.1 grid
     for (int i = 0; i < QBX_BOYS_ROWS; ++i) {
-        long double T = (long double)i * QBX_BOYS_DT;
-        long double e = expl(-T);
-        long double term = 1.0L / (2 * mtop + 1), sum = term;
-        for (int k = 1; k < 5000; ++k) {
-            term *= 2.0L * T / (2 * mtop + 2 * k + 1);
-            sum += term;
-            if (term < sum * 1e-21L) break;
-        }
-        long double F[QBX_BOYS_NCOL];
-        F[mtop] = e * sum;
-        for (int m = mtop - 1; m >= 0; --m) F[m] = (2.0L * T * F[m + 1] + e) / (2 * m + 1);
+        const long double e = boys_seq_ld((long double)i * QBX_BOYS_DT, QBX_BOYS_NCOL - 1, F);
         for (int m = 0; m < QBX_BOYS_NCOL; ++m) tab[(size_t)i * QBX_BOYS_NCOL + m] = (double)F[m];
         et[i] = (double)e;
+    }
+    // ERI kernels: per-order Taylor rows on the 0.25 grid
+    for (int i = 0; i < QBX_BOYSL_ROWS; ++i) {
+        const long double e = boys_seq_ld((long double)i * QBX_BOYSL_DT, 23, F);
         for (int L = 0; L <= QBX_BOYSL_LMAX; ++L) {
             double* r = &rows[((size_t)L * QBX_BOYSL_ROWS + i) * QBX_BOYSL_STRIDE];
             long double fact = 1.0L;
-            for (int k = 0; k < QBX_BOYS_TAYLOR; ++k) {
+            for (int k = 0; k < QBX_BOYSL_TAYLOR; ++k) {
                 if (k > 0) fact *= k;
                 r[k] = (double)(F[L + k] / fact);
             }
-            r[QBX_BOYS_TAYLOR] = (double)e;
+            r[QBX_BOYSL_TAYLOR] = (double)e;
         }
     }
 }
@@ -691,6 +705,7 @@ struct TaskCounts {
     std::vector<int64_t> p64, p32;
     std::vector<int> k64, k32;
     std::vector<int> w64, w32;   // waves of 64 kets per bra, prefix sums
+    std::vector<int> s64, s32;   // K-row blocks per bra, prefix sums
 };
 
 void task_counts(const qbx_basis* b, int cls, double thr, int mode, double cut, TaskCounts& tc)
@@ -705,6 +720,8 @@ void task_counts(const qbx_basis* b, int cls, double thr, int mode, double cut, 
     tc.k32.assign(std::max(nx, 1), 0);
     tc.w64.assign(nx + 1, 0);
     tc.w32.assign(nx + 1, 0);
+    tc.s64.assign(nx + 1, 0);
+    tc.s32.assign(nx + 1, 0);
     for (int i = 0; i < nx; ++i) {
         const double qi = PX.Q[i];
         int cnt;
@@ -722,6 +739,8 @@ void task_counts(const qbx_basis* b, int cls, double thr, int mode, double cut, 
         tc.p32[i + 1] = tc.p32[i] + (cnt - c64);
         tc.w64[i + 1] = tc.w64[i] + (c64 + 63) / 64;
         tc.w32[i + 1] = tc.w32[i] + (cnt - c64 + 63) / 64;
+        tc.s64[i + 1] = tc.s64[i] + ((c64 + 63) / 64 + ROW_CHUNKS - 1) / ROW_CHUNKS;
+        tc.s32[i + 1] = tc.s32[i] + ((cnt - c64 + 63) / 64 + ROW_CHUNKS - 1) / ROW_CHUNKS;
     }
 }
 
@@ -737,12 +756,16 @@ void ensure_tasks(qbx_basis* b, double thr, int mode, double cut)
         b->t64[cls].prefix.upload(tc.p64, st);
         b->t64[cls].koff.upload(tc.k64, st);
         b->t64[cls].cpre.upload(tc.w64, st);
+        b->t64[cls].spre.upload(tc.s64, st);
+        b->t64[cls].nsuper = tc.s64[nx];
         b->t64[cls].total = tc.p64[nx];
         b->t64[cls].nwave = tc.w64[nx];
         b->t64[cls].nbra = nx;
         b->t32[cls].prefix.upload(tc.p32, st);
         b->t32[cls].koff.upload(tc.k32, st);
         b->t32[cls].cpre.upload(tc.w32, st);
+        b->t32[cls].spre.upload(tc.s32, st);
+        b->t32[cls].nsuper = tc.s32[nx];
         b->t32[cls].total = tc.p32[nx];
         b->t32[cls].nwave = tc.w32[nx];
         b->t32[cls].nbra = nx;
@@ -873,6 +896,9 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
         a.prefix = t64.prefix.p;
         a.koff = t64.koff.p;
         a.cpre = t64.cpre.p;
+        a.spre = t64.spre.p;
+        a.nsuper = t64.nsuper;
+        a.lds_max = ctx->lds_max;
         a.total = t64.total;
         a.nwave = t64.nwave;
         a.nbra = t64.nbra;
@@ -883,6 +909,8 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
             a.prefix = t32.prefix.p;
             a.koff = t32.koff.p;
             a.cpre = t32.cpre.p;
+            a.spre = t32.spre.p;
+            a.nsuper = t32.nsuper;
             a.total = t32.total;
             a.nwave = t32.nwave;
             a.nbra = t32.nbra;
@@ -953,6 +981,7 @@ int qbx_init(int32_t device, qbx_ctx** out)
         c->boys.upload(tab, c->stream);
         c->bexp.upload(et, c->stream);
         c->boysl.upload(rows, c->stream);
+        QBX_HIP(hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
         // Per-XCD J/K accumulation needs the XCC_ID register to name each
         // block's XCD: check that it reads ids below QBX_MAX_XCD and that
         // several XCDs show up (a single-XCD device keeps one copy).

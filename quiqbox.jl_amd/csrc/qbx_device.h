@@ -18,12 +18,16 @@
 #define QBX_BOYS_ROWS 602
 #define QBX_BOYS_NCOL 17     // F_0 .. F_16: highest order 8 plus 8 Taylor terms
 #define QBX_BOYS_TAYLOR 8
-// Per-order rows for the ERI kernels (staged in LDS): for total angular
-// momentum L, row i holds F_{L+k}(T_i) / k! for k = 0..7, then exp(-T_i) and a
-// zero pad (80 bytes, 16-byte aligned rows).
-#define QBX_BOYSL_ROWS 602
-#define QBX_BOYSL_STRIDE 10
-#define QBX_BOYSL_LMAX 8
+// Per-order rows for the ERI kernels (staged in LDS, 23 KB): on the grid
+// T_i = i/4 (T < QBX_BOYS_TMAX), for total angular momentum L, row i holds
+// F_{L+k}(T_i) / k! for k = 0..10 and exp(-T_i) (96-byte rows).  With
+// |T - T_i| <= 1/8 the 11-term Taylor series truncates below 3e-18 relative.
+#define QBX_BOYSL_DT 0.25
+#define QBX_BOYSL_INVDT 4.0
+#define QBX_BOYSL_ROWS 242
+#define QBX_BOYSL_TAYLOR 11
+#define QBX_BOYSL_STRIDE 12
+#define QBX_BOYSL_LMAX 9
 
 #define QBX_MAXL 2           // largest shell angular momentum in the kernels
 #define QBX_MAXC 6           // Cartesian components of the largest shell

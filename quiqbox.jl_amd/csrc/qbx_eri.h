@@ -55,24 +55,27 @@ struct JKSink {
 };
 
 // Threads per block: classes with few accumulators (and so few registers) run
-// 8 or 16 waves per block, so that one LDS copy of the Boys rows (48 KB, three
+// 8 or 16 waves per block, so that one LDS copy of the Boys rows (23 KB, six
 // copies fit a CU) serves more waves.
 __host__ __device__ constexpr int eri_tpb(int nacc) { return nacc <= 3 ? 1024 : nacc <= 18 ? 512 : 256; }
 
 // Boys F_0..F_L from the LDS rows of order L (see qbx_device.h for the table):
-// an 8-term Taylor expansion of F_L about the nearest grid point, e^{-T} from
-// e^{-T_i} times the Taylor series of e^{T_i - T} (|T_i - T| <= 0.05, terms to
-// 8th order), the downward recurrence for the lower orders, and the asymptotic
-// form above the table.
+// an 11-term Taylor expansion of F_L about the nearest grid point, e^{-T} from
+// e^{-T_i} times the Taylor series of e^{T_i - T} (|T_i - T| <= 1/8, terms to
+// 11th order), the downward recurrence for the lower orders, and the
+// asymptotic form above the table.
 template <int L>
 __device__ __forceinline__ void boys_lds(double T, double* __restrict__ F, const double* __restrict__ sb)
 {
     if (T < QBX_BOYS_TMAX) {
-        const int i = __double2int_rn(T * QBX_BOYS_INVDT);
-        const double md = (double)i * QBX_BOYS_DT - T;
+        const int i = __double2int_rn(T * QBX_BOYSL_INVDT);
+        const double md = (double)i * QBX_BOYSL_DT - T;
         const double2* row = (const double2*)(sb + i * QBX_BOYSL_STRIDE);
-        const double2 c01 = row[0], c23 = row[1], c45 = row[2], c67 = row[3];
-        double acc = fma(md, c67.y, c67.x);
+        const double2 c01 = row[0], c23 = row[1], c45 = row[2], c67 = row[3], c89 = row[4], c10e = row[5];
+        double acc = fma(md, c10e.x, c89.y);
+        acc = fma(md, acc, c89.x);
+        acc = fma(md, acc, c67.y);
+        acc = fma(md, acc, c67.x);
         acc = fma(md, acc, c45.y);
         acc = fma(md, acc, c45.x);
         acc = fma(md, acc, c23.y);
@@ -81,8 +84,10 @@ __device__ __forceinline__ void boys_lds(double T, double* __restrict__ F, const
         acc = fma(md, acc, c01.x);
         F[L] = acc;
         if (L > 0) {
-            const double ei = sb[i * QBX_BOYSL_STRIDE + QBX_BOYS_TAYLOR];
-            double ex = fma(md, 1.0 / 40320.0, 1.0 / 5040.0);
+            double ex = fma(md, 1.0 / 39916800.0, 1.0 / 3628800.0);
+            ex = fma(md, ex, 1.0 / 362880.0);
+            ex = fma(md, ex, 1.0 / 40320.0);
+            ex = fma(md, ex, 1.0 / 5040.0);
             ex = fma(md, ex, 1.0 / 720.0);
             ex = fma(md, ex, 1.0 / 120.0);
             ex = fma(md, ex, 1.0 / 24.0);
@@ -90,7 +95,7 @@ __device__ __forceinline__ void boys_lds(double T, double* __restrict__ F, const
             ex = fma(md, ex, 0.5);
             ex = fma(md, ex, 1.0);
             ex = fma(md, ex, 1.0);
-            const double e = ei * ex;
+            const double e = c10e.y * ex;
             const double t2 = 2.0 * T;
 #pragma unroll
             for (int m = L - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
@@ -396,6 +401,197 @@ __global__ __launch_bounds__(eri_tpb(EriClass<LA, LB, LC, LD, R>::NACC)) void er
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// J/K with the bra's K rows in LDS.  A block of ROW_WAVES waves takes one bra
+// and up to ROW_CHUNKS of its 64-ket chunks.  Every K update of a quartet has
+// one index on the bra (K_ac, K_ad on the rows of A's functions; K_bc, K_bd on
+// B's), so the block sums them in LDS rows [set][A and B functions][N] with
+// LDS atomics and sends each nonzero row element to J/K once, instead of one
+// global atomic per quartet and element.  J_ab is reduced across the wave and
+// J_cd goes out per lane, as in eri_kernel.
+// ---------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ __launch_bounds__(64 * ROW_WAVES) void eri_jk_rows_kernel(EriArgs a)
+{
+    using Cls = EriClass<LA, LB, LC, LD, R>;
+    constexpr int TPB = 64 * ROW_WAVES;
+    constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB), NC = qbx_ncart(LC), ND = qbx_ncart(LD);
+    constexpr int LT = LA + LB + LC + LD;
+    constexpr int NCD = NC * ND;
+    constexpr int NROW = NA + NB;
+    constexpr bool PREFETCH = Cls::NACC <= 18;
+    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
+    extern __shared__ double krow[];    // [nset][NROW][N]
+    const int Ni = a.N;
+    const size_t N = (size_t)a.N;
+    const int nrowel = a.nset * NROW * Ni;
+    stage_boys<TPB>(sboys, a.boysl);
+    for (int i = threadIdx.x; i < nrowel; i += TPB) krow[i] = 0.0;
+    __syncthreads();
+    const double* sb = (const double*)sboys;
+
+    const int lane = (int)threadIdx.x & 63;
+    const int wib = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    // bra and superchunk of this block
+    const cptr<int> spre = cst(a.spre);
+    const int task = (int)blockIdx.x;
+    int lo = 0, hi = a.nbra;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (spre[mid] <= task) lo = mid; else hi = mid;
+    }
+    const int bi = lo;
+    const int sc = task - spre[bi];
+    const int nk = (int)(cst(a.prefix)[bi + 1] - cst(a.prefix)[bi]);
+    const int k0 = cst(a.koff)[bi];
+    const int sA = cst(a.bra.shA)[bi], sB = cst(a.bra.shB)[bi];
+    const cptr<DevShell> SA = cst(a.sh + sA);
+    const cptr<DevShell> SB = cst(a.sh + sB);
+    const cptr<double> ABp = cst(a.bra.AB + 3 * bi);
+    const R ABx = (R)ABp[0], ABy = (R)ABp[1], ABz = (R)ABp[2];
+    int fa[NA], fb[NB];
+#pragma unroll
+    for (int n = 0; n < NA; ++n) fa[n] = SA->bf[n] < 0 ? 0 : SA->bf[n];
+#pragma unroll
+    for (int n = 0; n < NB; ++n) fb[n] = SB->bf[n] < 0 ? 0 : SB->bf[n];
+    const JKSink sink(a);
+
+    for (int r = 0; r < ROW_CHUNKS / ROW_WAVES; ++r) {
+        const int chunk = (sc * (ROW_CHUNKS / ROW_WAVES) + r) * ROW_WAVES + wib;
+        if (chunk * 64 >= nk) break;
+        const int pos = chunk * 64 + lane;
+        const bool inrange = pos < nk;
+        const int ki = k0 + (inrange ? pos : 0);
+        const int sC = a.ket.shA[ki], sD = a.ket.shB[ki];
+        int kn = inrange ? a.ket.ppn[ki] : 0;
+        if (a.pdmax) {
+            const double* dm = a.pdmax;
+            const int ns = a.nshell;
+            const double dab = cst(dm)[sA * ns + sB];
+            double d = fmax(fmax(dab, dm[sC * ns + sD]), fmax(dm[sA * ns + sC], dm[sA * ns + sD]));
+            d = fmax(d, fmax(dm[sB * ns + sC], dm[sB * ns + sD]));
+            if (cst(a.bra.Q)[bi] * a.ket.Q[ki] * d < a.dthr) kn = 0;
+        }
+        if (__ballot(kn > 0) == 0) continue;
+        const bool live = kn > 0;
+
+        R acc[Cls::NACC];
+#pragma unroll
+        for (int n = 0; n < Cls::NACC; ++n) acc[n] = (R)0;
+        prim_loop<Cls, LT, R, PREFETCH, -1>(acc, cst(a.pp + cst(a.bra.ppo)[bi]), cst(a.bra.ppn)[bi],
+                                            a.pp + a.ket.ppo[ki], kn, sb);
+        const double* CDp = a.ket.AB + 3 * ki;
+        R kacc[Cls::NE * NCD];
+        Cls::hrr_ket(acc, kacc, (R)CDp[0], (R)CDp[1], (R)CDp[2]);
+
+        const DevShell* SC = a.sh + sC;
+        const DevShell* SD = a.sh + sD;
+        int fc[NC], fd[ND];
+        double scd[NCD];
+#pragma unroll
+        for (int n = 0; n < NC; ++n) fc[n] = SC->bf[n] < 0 ? 0 : SC->bf[n];
+#pragma unroll
+        for (int n = 0; n < ND; ++n) fd[n] = SD->bf[n] < 0 ? 0 : SD->bf[n];
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) scd[ic * ND + id] = SC->sc[ic] * SD->sc[id];
+        double f = live ? 1.0 : 0.0;
+        if (sA == sB) f *= 0.5;
+        if (sC == sD) f *= 0.5;
+        if (a.same && bi == ki) f *= 0.5;
+        const double fj = 4.0 * f, fk = 2.0 * f;
+
+        for (int s = 0; s < a.nset; ++s) {
+            const double* __restrict__ DJ = a.D + (size_t)s * N * N;
+            const double* __restrict__ DK = a.D + (size_t)(a.nset + s) * N * N;
+            double* Js = sink.J + (size_t)s * N * N;
+            double* rows = krow + s * NROW * Ni;
+            double dcd[NCD], dbc[NB * NC], dbd[NB * ND];
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = DJ[fc[ic] * N + fd[id]];
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic) dbc[ib * NC + ic] = DK[fb[ib] * N + fc[ic]];
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+                for (int id = 0; id < ND; ++id) dbd[ib * ND + id] = DK[fb[ib] * N + fd[id]];
+            double jcd[NCD], kbd[NB * ND], kbc[NB * NC];
+#pragma unroll
+            for (int n = 0; n < NCD; ++n) jcd[n] = 0.0;
+#pragma unroll
+            for (int n = 0; n < NB * ND; ++n) kbd[n] = 0.0;
+#pragma unroll
+            for (int n = 0; n < NB * NC; ++n) kbc[n] = 0.0;
+#pragma unroll
+            for (int ia = 0; ia < NA; ++ia) {
+                R out[NB * NCD];
+                Cls::hrr_bra(ia, kacc, out, ABx, ABy, ABz);
+                double dac[NC], dad[ND], kac[NC], kad[ND];
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic) { dac[ic] = DK[fa[ia] * N + fc[ic]]; kac[ic] = 0.0; }
+#pragma unroll
+                for (int id = 0; id < ND; ++id) { dad[id] = DK[fa[ia] * N + fd[id]]; kad[id] = 0.0; }
+#pragma unroll
+                for (int ib = 0; ib < NB; ++ib) {
+                    const double sab = SA->sc[ia] * SB->sc[ib];
+                    const double dab = cst(DJ)[fa[ia] * N + fb[ib]];
+                    double jab = 0.0;
+#pragma unroll
+                    for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                        for (int id = 0; id < ND; ++id) {
+                            const int cd = ic * ND + id;
+                            const double v = (double)out[ib * NCD + cd] * (sab * scd[cd]);
+                            jab += v * dcd[cd];
+                            jcd[cd] += v * dab;
+                            kad[id] += v * dbc[ib * NC + ic];
+                            kac[ic] += v * dbd[ib * ND + id];
+                            kbd[ib * ND + id] += v * dac[ic];
+                            kbc[ib * NC + ic] += v * dad[id];
+                        }
+                    jab = wave_sum(fj * jab);
+                    if (lane == 0) sink.add(Js + fa[ia] * N + fb[ib], jab);
+                }
+                if (live) {
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) atomicAdd(rows + ia * Ni + fd[id], fk * kad[id]);
+#pragma unroll
+                    for (int ic = 0; ic < NC; ++ic) atomicAdd(rows + ia * Ni + fc[ic], fk * kac[ic]);
+                }
+            }
+            if (live) {
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) sink.add(Js + fc[ic] * N + fd[id], fj * jcd[ic * ND + id]);
+#pragma unroll
+                for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) atomicAdd(rows + (NA + ib) * Ni + fd[id], fk * kbd[ib * ND + id]);
+#pragma unroll
+                for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+                    for (int ic = 0; ic < NC; ++ic) atomicAdd(rows + (NA + ib) * Ni + fc[ic], fk * kbc[ib * NC + ic]);
+            }
+        }
+    }
+    __syncthreads();
+    // each nonzero row element to K once
+    for (int i = threadIdx.x; i < nrowel; i += TPB) {
+        const double v = krow[i];
+        if (v == 0.0) continue;
+        const int s = i / (NROW * Ni), rem = i - s * NROW * Ni;
+        const int row = rem / Ni, col = rem - row * Ni;
+        const int fr = row < NA ? max(SA->bf[row], 0) : max(SB->bf[row - NA], 0);
+        sink.add(sink.K + (size_t)s * N * N + (size_t)fr * N + col, v);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Staged classes: the [e0|f0] set is too large for the register file, so each
@@ -711,6 +907,33 @@ void launch_eri(const EriArgs& a, hipStream_t s)
                            dim3(64 * STAGE_WAVES_PER_BLOCK), 0, s, a);
         QBX_HIP(hipGetLastError());
     } else {
+        if constexpr (MODE == MODE_JK) {
+            const size_t shm = (size_t)a.nset * (qbx_ncart(LA) + qbx_ncart(LB)) * a.N * sizeof(double);
+            const size_t boys_lds = QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE * sizeof(double);
+            // the attribute call fails where the device cannot give a block
+            // that much LDS; the one-wave-per-chunk kernel runs then
+            static const bool rows_ok = [] {
+                const bool ok = hipFuncSetAttribute((const void*)eri_jk_rows_kernel<LA, LB, LC, LD, R>,
+                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                    ROW_LDS_MAX) == hipSuccess;
+                (void)hipGetLastError();
+                return ok;
+            }();
+            if (rows_ok && a.spre && shm <= (size_t)ROW_LDS_MAX && shm + boys_lds <= (size_t)a.lds_max) {
+                if (a.nsuper <= 0) return;
+                static const size_t scratch = [] {
+                    hipFuncAttributes fa;
+                    if (hipFuncGetAttributes(&fa, (const void*)eri_jk_rows_kernel<LA, LB, LC, LD, R>) != hipSuccess)
+                        return (size_t)1;
+                    return (size_t)fa.localSizeBytes;
+                }();
+                hipStream_t use = (scratch > 0 && a.scratch_stream) ? (hipStream_t)a.scratch_stream : s;
+                hipLaunchKernelGGL((eri_jk_rows_kernel<LA, LB, LC, LD, R>), dim3((unsigned)a.nsuper),
+                                   dim3(64 * ROW_WAVES), shm, use, a);
+                QBX_HIP(hipGetLastError());
+                return;
+            }
+        }
         if (a.nwave <= 0) return;
         // Kernels that spill to scratch run on the caller's main stream only:
         // each extra hardware queue would need its own scratch allocation.

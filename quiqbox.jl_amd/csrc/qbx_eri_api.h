@@ -7,6 +7,16 @@
 namespace qbx {
 enum EriMode { MODE_JK = 0, MODE_DENSE = 1, MODE_SCHWARZ = 2 };
 
+// J/K kernel with the bra's K rows in LDS: ROW_WAVES waves per block, up to
+// ROW_CHUNKS chunks of 64 kets of one bra per block, at most ROW_LDS_MAX bytes
+// of rows (larger bases use the one-wave-per-chunk kernel).
+#ifndef QBX_ROW_WAVES
+#define QBX_ROW_WAVES 4
+#endif
+constexpr int ROW_WAVES = QBX_ROW_WAVES;
+constexpr int ROW_CHUNKS = 16;
+constexpr int ROW_LDS_MAX = 64 * 1024;
+
 struct PairClassDev {
     int n;
     const int* shA;
@@ -27,6 +37,9 @@ struct EriArgs {
     const int64_t* prefix;    // [nbra+1] prefix over bras of the screened ket counts
     const int* koff;          // [nbra] first ket of each bra
     const int* cpre;          // [nbra+1] prefix over bras of ceil(kets / 64): waves
+    const int* spre;          // [nbra+1] prefix over bras of ceil(waves / ROW_CHUNKS): K-row blocks
+    int nsuper;               // number of K-row blocks
+    int lds_max;              // LDS bytes a block may use on this device
     int64_t total;            // Schwarz mode: number of diagonal pairs
     int nwave;                // J/K and dense modes: number of waves
     int nbra;
