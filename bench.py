@@ -183,7 +183,7 @@ def learned_section(nuc, gb, res64, t64, n_train, epochs):
     t_data = time.perf_counter() - t0
     model = lg.LearnedGuess(seed=0)
     t0 = time.perf_counter()
-    hist = model.train(data, epochs=epochs, batch=2048, lr=1e-3, log=log)
+    hist = model.train(data, epochs=epochs, batch=512, lr=1e-3, log=log)
     t_train = time.perf_counter() - t0
     strat = qb.SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-8))
     out = {"training_systems": len(data), "training_pairs": int(sum(len(x) for x, _ in data)),

@@ -402,6 +402,44 @@ __global__ __launch_bounds__(eri_tpb(EriClass<LA, LB, LC, LD, R>::NACC)) void er
 }
 
 
+// K destinations of the staged digest: straight to the (per-XCD) K matrix,
+// or to the block's LDS rows of the bra's functions (row = a component, or
+// NA + b component).
+struct KToGlobal {
+    const JKSink& sink;
+    size_t N;
+    __device__ __forceinline__ void add(int s, int fn, int, int col, double v) const
+    {
+        sink.add(sink.K + (size_t)s * N * N + (size_t)fn * N + col, v);
+    }
+};
+template <int NROW> struct KToRows {
+    double* rows;
+    int Ni;
+    __device__ __forceinline__ void add(int s, int, int lrow, int col, double v) const
+    {
+        atomicAdd(rows + (s * NROW + lrow) * Ni + col, v);
+    }
+};
+
+// Flush of a block's K rows ([nset][NA + NB][N] in LDS) to K: each nonzero
+// element once; with `clear` the rows are zeroed for the block's next bra.
+template <int NA, int TPB>
+__device__ __forceinline__ void flush_rows(double* krow, int nrowel, int nrow, int Ni, const int* bfA,
+                                           const int* bfB, const JKSink& sink, bool clear)
+{
+    const size_t N = (size_t)Ni;
+    for (int i = threadIdx.x; i < nrowel; i += TPB) {
+        const double v = krow[i];
+        if (v == 0.0) continue;
+        if (clear) krow[i] = 0.0;
+        const int s = i / (nrow * Ni), rem = i - s * nrow * Ni;
+        const int row = rem / Ni, col = rem - row * Ni;
+        const int fr = row < NA ? max(bfA[row], 0) : max(bfB[row - NA], 0);
+        sink.add(sink.K + (size_t)s * N * N + (size_t)fr * N + col, v);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // J/K with the bra's K rows in LDS.  A block of ROW_WAVES waves takes one bra
 // and up to ROW_CHUNKS of its 64-ket chunks.  Every K update of a quartet has
@@ -582,15 +620,7 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void eri_jk_rows_kernel(EriArgs a)
         }
     }
     __syncthreads();
-    // each nonzero row element to K once
-    for (int i = threadIdx.x; i < nrowel; i += TPB) {
-        const double v = krow[i];
-        if (v == 0.0) continue;
-        const int s = i / (NROW * Ni), rem = i - s * NROW * Ni;
-        const int row = rem / Ni, col = rem - row * Ni;
-        const int fr = row < NA ? max(SA->bf[row], 0) : max(SB->bf[row - NA], 0);
-        sink.add(sink.K + (size_t)s * N * N + (size_t)fr * N + col, v);
-    }
+    flush_rows<NA, TPB>(krow, nrowel, NROW, Ni, a.sh[sA].bf, a.sh[sB].bf, sink, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -656,15 +686,184 @@ __device__ __forceinline__ void staged_groups(std::integer_sequence<int, Gs...>,
     (staged_group<Cls, LT, R, Gs>(BP, bn, KP, kn, sb, CDx, CDy, CDz, ws), ...);
 }
 
+// One wave task of a staged class: bra `bi`, kets chunk*64 .. chunk*64+63 of
+// its list.  Evaluates the groups into the wave's workspace slot, then writes
+// the dense tensor or digests J/K (J through the sink, K through `kout`).
+template <int LA, int LB, int LC, int LD, typename R, int MODE, class KOut>
+__device__ __forceinline__ void staged_chunk(const EriArgs& a, int bi, int chunk, int lane, const WsSlot<R>& ws,
+                                             const double* sb, const KOut& kout)
+{
+    using Cls = EriClass<LA, LB, LC, LD, R>;
+    constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB), NC = qbx_ncart(LC), ND = qbx_ncart(LD);
+    constexpr int LT = LA + LB + LC + LD;
+    constexpr int NCD = NC * ND;
+    const size_t N = (size_t)a.N;
+    const int nk = (int)(cst(a.prefix)[bi + 1] - cst(a.prefix)[bi]);
+    const int pos = chunk * 64 + lane;
+    const bool inrange = pos < nk;
+    const int ki = cst(a.koff)[bi] + (inrange ? pos : 0);
+    const int sA = cst(a.bra.shA)[bi], sB = cst(a.bra.shB)[bi];
+    const int sC = a.ket.shA[ki], sD = a.ket.shB[ki];
+    int kn = inrange ? a.ket.ppn[ki] : 0;
+    if (MODE == MODE_JK && a.pdmax) {
+        const double* dm = a.pdmax;
+        const int ns = a.nshell;
+        const double dab = cst(dm)[sA * ns + sB];
+        double d = fmax(fmax(dab, dm[sC * ns + sD]), fmax(dm[sA * ns + sC], dm[sA * ns + sD]));
+        d = fmax(d, fmax(dm[sB * ns + sC], dm[sB * ns + sD]));
+        if (cst(a.bra.Q)[bi] * a.ket.Q[ki] * d < a.dthr) kn = 0;
+    }
+    if (__ballot(kn > 0) == 0) return;
+    const bool live = kn > 0;
+
+    const double* CDp = a.ket.AB + 3 * ki;
+    staged_groups<Cls, LT, R>(std::make_integer_sequence<int, Cls::NGRP>{},
+                              cst(a.pp + cst(a.bra.ppo)[bi]), cst(a.bra.ppn)[bi],
+                              a.pp + a.ket.ppo[ki], kn, sb, (R)CDp[0], (R)CDp[1], (R)CDp[2], ws);
+
+    const cptr<double> ABp = cst(a.bra.AB + 3 * bi);
+    const R ABx = (R)ABp[0], ABy = (R)ABp[1], ABz = (R)ABp[2];
+    const cptr<DevShell> SA = cst(a.sh + sA);
+    const cptr<DevShell> SB = cst(a.sh + sB);
+    const DevShell* SC = a.sh + sC;
+    const DevShell* SD = a.sh + sD;
+    int fc[NC], fd[ND];
+    double scc[NC], scdd[ND];
+#pragma unroll
+    for (int n = 0; n < NC; ++n) { fc[n] = SC->bf[n]; scc[n] = SC->sc[n]; }
+#pragma unroll
+    for (int n = 0; n < ND; ++n) { fd[n] = SD->bf[n]; scdd[n] = SD->sc[n]; }
+
+    if (MODE == MODE_DENSE) {
+        if (!live) return;
+#pragma unroll 1
+        for (int iab = 0; iab < NA * NB; ++iab) {
+            const int ia = iab / NB, ib = iab - ia * NB;
+            R out[NCD];
+            Cls::hrr_bra_ab(iab, ws, out, ABx, ABy, ABz);
+            const int fa_ = SA->bf[ia], fb_ = SB->bf[ib];
+            if (fa_ < 0 || fb_ < 0) continue;
+            const double sab = SA->sc[ia] * SB->sc[ib];
+            const size_t i = fa_, j = fb_;
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                for (int id = 0; id < ND; ++id) {
+                    if (fc[ic] < 0 || fd[id] < 0) continue;
+                    const size_t k = fc[ic], l = fd[id];
+                    const double v = (double)out[ic * ND + id] * sab * (scc[ic] * scdd[id]);
+                    a.eri[((i * N + j) * N + k) * N + l] = v;
+                    a.eri[((j * N + i) * N + k) * N + l] = v;
+                    a.eri[((i * N + j) * N + l) * N + k] = v;
+                    a.eri[((j * N + i) * N + l) * N + k] = v;
+                    a.eri[((k * N + l) * N + i) * N + j] = v;
+                    a.eri[((l * N + k) * N + i) * N + j] = v;
+                    a.eri[((k * N + l) * N + j) * N + i] = v;
+                    a.eri[((l * N + k) * N + j) * N + i] = v;
+                }
+        }
+        return;
+    }
+
+    // ---- J/K digest from the (a b| rows, in two passes that each keep
+    //      few sums live: A) J_ab, K_ac, K_ad per row; B) J_cd summed over
+    //      all rows, K_bd and K_bc summed over a ----
+    double f = live ? 1.0 : 0.0;
+    if (sA == sB) f *= 0.5;
+    if (sC == sD) f *= 0.5;
+    if (a.same && bi == ki) f *= 0.5;
+    const double fj = 4.0 * f, fk = 2.0 * f;
+#pragma unroll
+    for (int n = 0; n < NC; ++n) fc[n] = fc[n] < 0 ? 0 : fc[n];
+#pragma unroll
+    for (int n = 0; n < ND; ++n) fd[n] = fd[n] < 0 ? 0 : fd[n];
+    const JKSink sink(a);
+    for (int s = 0; s < a.nset; ++s) {
+        const double* __restrict__ DJ = a.D + (size_t)s * N * N;
+        const double* __restrict__ DK = a.D + (size_t)(a.nset + s) * N * N;
+        double* Js = sink.J + (size_t)s * N * N;
+#pragma unroll 1
+        for (int iab = 0; iab < NA * NB; ++iab) {
+            const int ia = iab / NB, ib = iab - ia * NB;
+            R out[NCD];
+            Cls::hrr_bra_ab(iab, ws, out, ABx, ABy, ABz);
+            const int fa_ = max(SA->bf[ia], 0), fb_ = max(SB->bf[ib], 0);
+            const double sab = SA->sc[ia] * SB->sc[ib];
+            double jab = 0.0, kac[NC], kad[ND];
+#pragma unroll
+            for (int n = 0; n < NC; ++n) kac[n] = 0.0;
+#pragma unroll
+            for (int n = 0; n < ND; ++n) kad[n] = 0.0;
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                for (int id = 0; id < ND; ++id) {
+                    const double v = (double)out[ic * ND + id] * sab * (scc[ic] * scdd[id]);
+                    jab += v * DJ[fc[ic] * N + fd[id]];
+                    kac[ic] += v * DK[fb_ * N + fd[id]];
+                    kad[id] += v * DK[fb_ * N + fc[ic]];
+                }
+            jab = wave_sum(fj * jab);
+            if (lane == 0) sink.add(Js + fa_ * N + fb_, jab);
+            if (live) {
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic) kout.add(s, fa_, ia, fc[ic], fk * kac[ic]);
+#pragma unroll
+                for (int id = 0; id < ND; ++id) kout.add(s, fa_, ia, fd[id], fk * kad[id]);
+            }
+        }
+        double jcd[NCD];
+#pragma unroll
+        for (int n = 0; n < NCD; ++n) jcd[n] = 0.0;
+#pragma unroll 1
+        for (int ib = 0; ib < NB; ++ib) {
+            const int fb_ = max(SB->bf[ib], 0);
+            double kbd[ND], kbc[NC];
+#pragma unroll
+            for (int n = 0; n < ND; ++n) kbd[n] = 0.0;
+#pragma unroll
+            for (int n = 0; n < NC; ++n) kbc[n] = 0.0;
+#pragma unroll 1
+            for (int ia = 0; ia < NA; ++ia) {
+                R out[NCD];
+                Cls::hrr_bra_ab(ia * NB + ib, ws, out, ABx, ABy, ABz);
+                const int fa_ = max(SA->bf[ia], 0);
+                const double sab = SA->sc[ia] * SB->sc[ib];
+                const double dab = cst(DJ)[fa_ * N + fb_];
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) {
+                        const int cd = ic * ND + id;
+                        const double v = (double)out[cd] * sab * (scc[ic] * scdd[id]);
+                        jcd[cd] += v * dab;
+                        kbd[id] += v * DK[fa_ * N + fc[ic]];
+                        kbc[ic] += v * DK[fa_ * N + fd[id]];
+                    }
+            }
+            if (live) {
+#pragma unroll
+                for (int id = 0; id < ND; ++id) kout.add(s, fb_, NA + ib, fd[id], fk * kbd[id]);
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic) kout.add(s, fb_, NA + ib, fc[ic], fk * kbc[ic]);
+            }
+        }
+        if (live) {
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                for (int id = 0; id < ND; ++id) sink.add(Js + fc[ic] * N + fd[id], fj * jcd[ic * ND + id]);
+        }
+    }
+}
+
 template <int LA, int LB, int LC, int LD, typename R, int MODE>
 __global__ __launch_bounds__(64 * STAGE_WAVES_PER_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void eri_staged_kernel(EriArgs a)
 {
     using Cls = EriClass<LA, LB, LC, LD, R>;
     constexpr int TPB = 64 * STAGE_WAVES_PER_BLOCK;
-    constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB), NC = qbx_ncart(LC), ND = qbx_ncart(LD);
-    constexpr int LT = LA + LB + LC + LD;
-    constexpr int NCD = NC * ND;
+    constexpr int NCD = qbx_ncart(LC) * qbx_ncart(LD);
     constexpr int WSV = Cls::NE * NCD;          // workspace values per lane
     __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
     stage_boys<TPB>(sboys, a.boysl);
@@ -676,7 +875,8 @@ void eri_staged_kernel(EriArgs a)
     const int nslot = (int)gridDim.x * STAGE_WAVES_PER_BLOCK;
     const WsSlot<R> ws((R*)a.ws + (size_t)slot * WSV * 64, WSV, lane);
     const cptr<int> cpre = cst(a.cpre);
-    const size_t N = (size_t)a.N;
+    const JKSink sink(a);
+    const KToGlobal kout{sink, (size_t)a.N};
 
     for (int wave = slot; wave < a.nwave; wave += nslot) {
         int lo = 0, hi = a.nbra;
@@ -684,165 +884,60 @@ void eri_staged_kernel(EriArgs a)
             const int mid = (lo + hi) >> 1;
             if (cpre[mid] <= wave) lo = mid; else hi = mid;
         }
+        staged_chunk<LA, LB, LC, LD, R, MODE>(a, lo, wave - cpre[lo], lane, ws, sb, kout);
+    }
+}
+
+// Staged J/K with the bra's K rows in LDS: a persistent grid of blocks, each
+// taking one (bra, up to ROW_CHUNKS chunks) task at a time; its waves share
+// the task's chunks, K goes to the rows, and the rows are flushed to K once
+// per task.
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ __launch_bounds__(64 * STAGE_WAVES_PER_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void eri_staged_rows_kernel(EriArgs a)
+{
+    using Cls = EriClass<LA, LB, LC, LD, R>;
+    constexpr int W = STAGE_WAVES_PER_BLOCK;
+    constexpr int TPB = 64 * W;
+    constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB);
+    constexpr int NROW = NA + NB;
+    constexpr int NCD = qbx_ncart(LC) * qbx_ncart(LD);
+    constexpr int WSV = Cls::NE * NCD;
+    static_assert(ROW_CHUNKS % W == 0, "chunks per task must split evenly over the waves");
+    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
+    extern __shared__ double krow[];    // [nset][NROW][N]
+    const int Ni = a.N;
+    const int nrowel = a.nset * NROW * Ni;
+    stage_boys<TPB>(sboys, a.boysl);
+    for (int i = threadIdx.x; i < nrowel; i += TPB) krow[i] = 0.0;
+    __syncthreads();
+    const double* sb = (const double*)sboys;
+
+    const int lane = (int)threadIdx.x & 63;
+    const int wib = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const WsSlot<R> ws((R*)a.ws + (size_t)((int)blockIdx.x * W + wib) * WSV * 64, WSV, lane);
+    const cptr<int> spre = cst(a.spre);
+    const JKSink sink(a);
+    const KToRows<NROW> kout{krow, Ni};
+
+    for (int task = (int)blockIdx.x; task < a.nsuper; task += (int)gridDim.x) {
+        int lo = 0, hi = a.nbra;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (spre[mid] <= task) lo = mid; else hi = mid;
+        }
         const int bi = lo;
+        const int sc = task - spre[bi];
         const int nk = (int)(cst(a.prefix)[bi + 1] - cst(a.prefix)[bi]);
-        const int pos = (wave - cpre[bi]) * 64 + lane;
-        const bool inrange = pos < nk;
-        const int ki = cst(a.koff)[bi] + (inrange ? pos : 0);
+        for (int r = 0; r < ROW_CHUNKS / W; ++r) {
+            const int chunk = (sc * (ROW_CHUNKS / W) + r) * W + wib;
+            if (chunk * 64 >= nk) break;
+            staged_chunk<LA, LB, LC, LD, R, MODE_JK>(a, bi, chunk, lane, ws, sb, kout);
+        }
+        __syncthreads();
         const int sA = cst(a.bra.shA)[bi], sB = cst(a.bra.shB)[bi];
-        const int sC = a.ket.shA[ki], sD = a.ket.shB[ki];
-        int kn = inrange ? a.ket.ppn[ki] : 0;
-        if (MODE == MODE_JK && a.pdmax) {
-            const double* dm = a.pdmax;
-            const int ns = a.nshell;
-            const double dab = cst(dm)[sA * ns + sB];
-            double d = fmax(fmax(dab, dm[sC * ns + sD]), fmax(dm[sA * ns + sC], dm[sA * ns + sD]));
-            d = fmax(d, fmax(dm[sB * ns + sC], dm[sB * ns + sD]));
-            if (cst(a.bra.Q)[bi] * a.ket.Q[ki] * d < a.dthr) kn = 0;
-        }
-        if (__ballot(kn > 0) == 0) continue;
-        const bool live = kn > 0;
-
-        const double* CDp = a.ket.AB + 3 * ki;
-        staged_groups<Cls, LT, R>(std::make_integer_sequence<int, Cls::NGRP>{},
-                                  cst(a.pp + cst(a.bra.ppo)[bi]), cst(a.bra.ppn)[bi],
-                                  a.pp + a.ket.ppo[ki], kn, sb, (R)CDp[0], (R)CDp[1], (R)CDp[2], ws);
-
-        const cptr<double> ABp = cst(a.bra.AB + 3 * bi);
-        const R ABx = (R)ABp[0], ABy = (R)ABp[1], ABz = (R)ABp[2];
-        const cptr<DevShell> SA = cst(a.sh + sA);
-        const cptr<DevShell> SB = cst(a.sh + sB);
-        const DevShell* SC = a.sh + sC;
-        const DevShell* SD = a.sh + sD;
-        int fc[NC], fd[ND];
-        double scc[NC], scdd[ND];
-#pragma unroll
-        for (int n = 0; n < NC; ++n) { fc[n] = SC->bf[n]; scc[n] = SC->sc[n]; }
-#pragma unroll
-        for (int n = 0; n < ND; ++n) { fd[n] = SD->bf[n]; scdd[n] = SD->sc[n]; }
-
-        if (MODE == MODE_DENSE) {
-            if (!live) continue;
-#pragma unroll 1
-            for (int iab = 0; iab < NA * NB; ++iab) {
-                const int ia = iab / NB, ib = iab - ia * NB;
-                R out[NCD];
-                Cls::hrr_bra_ab(iab, ws, out, ABx, ABy, ABz);
-                const int fa_ = SA->bf[ia], fb_ = SB->bf[ib];
-                if (fa_ < 0 || fb_ < 0) continue;
-                const double sab = SA->sc[ia] * SB->sc[ib];
-                const size_t i = fa_, j = fb_;
-#pragma unroll
-                for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-                    for (int id = 0; id < ND; ++id) {
-                        if (fc[ic] < 0 || fd[id] < 0) continue;
-                        const size_t k = fc[ic], l = fd[id];
-                        const double v = (double)out[ic * ND + id] * sab * (scc[ic] * scdd[id]);
-                        a.eri[((i * N + j) * N + k) * N + l] = v;
-                        a.eri[((j * N + i) * N + k) * N + l] = v;
-                        a.eri[((i * N + j) * N + l) * N + k] = v;
-                        a.eri[((j * N + i) * N + l) * N + k] = v;
-                        a.eri[((k * N + l) * N + i) * N + j] = v;
-                        a.eri[((l * N + k) * N + i) * N + j] = v;
-                        a.eri[((k * N + l) * N + j) * N + i] = v;
-                        a.eri[((l * N + k) * N + j) * N + i] = v;
-                    }
-            }
-            continue;
-        }
-
-        // ---- J/K digest from the (a b| rows, in two passes that each keep
-        //      few sums live: A) J_ab, K_ac, K_ad per row; B) J_cd summed over
-        //      all rows, K_bd and K_bc summed over a ----
-        double f = live ? 1.0 : 0.0;
-        if (sA == sB) f *= 0.5;
-        if (sC == sD) f *= 0.5;
-        if (a.same && bi == ki) f *= 0.5;
-        const double fj = 4.0 * f, fk = 2.0 * f;
-#pragma unroll
-        for (int n = 0; n < NC; ++n) fc[n] = fc[n] < 0 ? 0 : fc[n];
-#pragma unroll
-        for (int n = 0; n < ND; ++n) fd[n] = fd[n] < 0 ? 0 : fd[n];
-        const JKSink sink(a);
-        for (int s = 0; s < a.nset; ++s) {
-            const double* __restrict__ DJ = a.D + (size_t)s * N * N;
-            const double* __restrict__ DK = a.D + (size_t)(a.nset + s) * N * N;
-            double* Js = sink.J + (size_t)s * N * N;
-            double* Ks = sink.K + (size_t)s * N * N;
-#pragma unroll 1
-            for (int iab = 0; iab < NA * NB; ++iab) {
-                const int ia = iab / NB, ib = iab - ia * NB;
-                R out[NCD];
-                Cls::hrr_bra_ab(iab, ws, out, ABx, ABy, ABz);
-                const int fa_ = max(SA->bf[ia], 0), fb_ = max(SB->bf[ib], 0);
-                const double sab = SA->sc[ia] * SB->sc[ib];
-                double jab = 0.0, kac[NC], kad[ND];
-#pragma unroll
-                for (int n = 0; n < NC; ++n) kac[n] = 0.0;
-#pragma unroll
-                for (int n = 0; n < ND; ++n) kad[n] = 0.0;
-#pragma unroll
-                for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-                    for (int id = 0; id < ND; ++id) {
-                        const double v = (double)out[ic * ND + id] * sab * (scc[ic] * scdd[id]);
-                        jab += v * DJ[fc[ic] * N + fd[id]];
-                        kac[ic] += v * DK[fb_ * N + fd[id]];
-                        kad[id] += v * DK[fb_ * N + fc[ic]];
-                    }
-                jab = wave_sum(fj * jab);
-                if (lane == 0) sink.add(Js + fa_ * N + fb_, jab);
-                if (live) {
-#pragma unroll
-                    for (int ic = 0; ic < NC; ++ic) sink.add(Ks + fa_ * N + fc[ic], fk * kac[ic]);
-#pragma unroll
-                    for (int id = 0; id < ND; ++id) sink.add(Ks + fa_ * N + fd[id], fk * kad[id]);
-                }
-            }
-            double jcd[NCD];
-#pragma unroll
-            for (int n = 0; n < NCD; ++n) jcd[n] = 0.0;
-#pragma unroll 1
-            for (int ib = 0; ib < NB; ++ib) {
-                const int fb_ = max(SB->bf[ib], 0);
-                double kbd[ND], kbc[NC];
-#pragma unroll
-                for (int n = 0; n < ND; ++n) kbd[n] = 0.0;
-#pragma unroll
-                for (int n = 0; n < NC; ++n) kbc[n] = 0.0;
-#pragma unroll 1
-                for (int ia = 0; ia < NA; ++ia) {
-                    R out[NCD];
-                    Cls::hrr_bra_ab(ia * NB + ib, ws, out, ABx, ABy, ABz);
-                    const int fa_ = max(SA->bf[ia], 0);
-                    const double sab = SA->sc[ia] * SB->sc[ib];
-                    const double dab = cst(DJ)[fa_ * N + fb_];
-#pragma unroll
-                    for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-                        for (int id = 0; id < ND; ++id) {
-                            const int cd = ic * ND + id;
-                            const double v = (double)out[cd] * sab * (scc[ic] * scdd[id]);
-                            jcd[cd] += v * dab;
-                            kbd[id] += v * DK[fa_ * N + fc[ic]];
-                            kbc[ic] += v * DK[fa_ * N + fd[id]];
-                        }
-                }
-                if (live) {
-#pragma unroll
-                    for (int id = 0; id < ND; ++id) sink.add(Ks + fb_ * N + fd[id], fk * kbd[id]);
-#pragma unroll
-                    for (int ic = 0; ic < NC; ++ic) sink.add(Ks + fb_ * N + fc[ic], fk * kbc[ic]);
-                }
-            }
-            if (live) {
-#pragma unroll
-                for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-                    for (int id = 0; id < ND; ++id) sink.add(Js + fc[ic] * N + fd[id], fj * jcd[ic * ND + id]);
-            }
-        }
+        flush_rows<NA, TPB>(krow, nrowel, NROW, Ni, a.sh[sA].bf, a.sh[sB].bf, sink, true);
+        __syncthreads();
     }
 }
 
@@ -899,6 +994,33 @@ void launch_eri(const EriArgs& a, hipStream_t s)
     } else if constexpr (EriClass<LA, LB, LC, LD, R>::STAGED) {
         if (a.nwave <= 0) return;
         if (!a.ws) throw QbxError(QBX_EINTERNAL, "staged ERI class launched without a workspace");
+        // LDS K rows pay where K updates are dense: (NA+NB)(NC+ND) updates
+        // per quartet against NA NB NC ND integrals, at least 1 to 5.  The
+        // sparser (dd|dp) and (dd|dd) lose more to the block's shared task
+        // (waves idle at its end, fewer tasks in flight) than they save.
+        constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB), NC = qbx_ncart(LC), ND = qbx_ncart(LD);
+        constexpr bool DENSE_K = 5 * (NA + NB) * (NC + ND) >= NA * NB * NC * ND;
+        if constexpr (MODE == MODE_JK && DENSE_K) {
+            const size_t shm = (size_t)a.nset * (NA + NB) * a.N * sizeof(double);
+            const size_t boys_lds = QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE * sizeof(double);
+            static const bool rows_ok = [] {
+                const bool ok = hipFuncSetAttribute((const void*)eri_staged_rows_kernel<LA, LB, LC, LD, R>,
+                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                    ROW_LDS_MAX) == hipSuccess;
+                (void)hipGetLastError();
+                return ok;
+            }();
+            if (rows_ok && a.spre && shm <= (size_t)ROW_LDS_MAX && shm + boys_lds <= (size_t)a.lds_max) {
+                if (a.nsuper <= 0) return;
+                const int max_blocks = a.ws_slots / STAGE_WAVES_PER_BLOCK;
+                const int blocks = a.nsuper < max_blocks ? a.nsuper : max_blocks;
+                if (blocks <= 0) throw QbxError(QBX_EINTERNAL, "staged ERI workspace holds no block");
+                hipLaunchKernelGGL((eri_staged_rows_kernel<LA, LB, LC, LD, R>), dim3((unsigned)blocks),
+                                   dim3(64 * STAGE_WAVES_PER_BLOCK), shm, s, a);
+                QBX_HIP(hipGetLastError());
+                return;
+            }
+        }
         const int slots = a.nwave < a.ws_slots ? a.nwave : a.ws_slots;
         const int blocks = (slots + STAGE_WAVES_PER_BLOCK - 1) / STAGE_WAVES_PER_BLOCK;
         if (blocks * STAGE_WAVES_PER_BLOCK > a.ws_slots)
