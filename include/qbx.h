@@ -30,6 +30,7 @@ extern "C" {
 #define QBX_ENOMEM (-3)   /* allocation failure                                 */
 #define QBX_EUNSUP (-4)   /* unsupported basis (angular momentum above the build) */
 #define QBX_EINTERNAL (-5)
+#define QBX_ENOCONV (-6)  /* an iteration did not settle (qbx_density_purify)  */
 
 #define QBX_PREC_FP64 0   /* every quartet in FP64                              */
 #define QBX_PREC_MIXED 1  /* FP32 primitive arithmetic for quartets whose Schwarz
@@ -155,6 +156,29 @@ int  qbx_basis_shells(const qbx_basis* basis, int32_t* n_shell, int32_t* n_radia
 int  qbx_rhf_gradient(qbx_ctx* ctx, qbx_basis* basis, const double* Dt, const double* W, int32_t n_nuc,
                       const double* Z, const double* R, double schwarz_thr, double* g_xpn, double* g_lin,
                       double* g_cen);
+
+/* ---- Density from the Fock matrix without diagonalization ----------------
+   The reference forms each SCF step's density from eigen(X' F X)
+   (solveFockMatrix/getC/getD, HartreeFock.jl:46-59, 296-302); this entry point
+   gives the same matrix, D = C[:, :nocc] C[:, :nocc]^T, as the spectral
+   projector of X' F X onto its nocc lowest eigenvalues, by trace-correcting
+   purification (SP2: P <- P^2 or 2P - P^2 from P0 = (hi I - X'FX)/(hi - lo),
+   Gershgorin bounds lo, hi), every product an FP64 GEMM on the matrix cores.
+   Host arrays, n x n row-major; X is the orthonormalization
+   (HartreeFock.jl:39-42).  Needs a gap between eigenvalues nocc and nocc+1:
+   QBX_ENOCONV when the trace or the idempotency has not settled after
+   max_iter products (the caller then diagonalizes).  iterations (products
+   taken) and idempotency (|tr P - tr P^2| at the end) may be NULL. */
+int  qbx_density_purify(qbx_ctx* ctx, int32_t n, int32_t nocc, const double* X, const double* F, double* D,
+                        int32_t max_iter, int32_t* iterations, double* idempotency);
+
+/* C[M x N] = alpha op(A) op(B) + beta C in FP64 on the matrix cores
+   (v_mfma_f64_16x16x4f64), row-major device matrices, op as for
+   qbx_gemm_bf16; enqueued on `stream` (NULL: default stream), no sync.  The
+   GEMM of the purification above, exported for its tests. */
+int  qbx_gemm_f64(int32_t trans_a, int32_t trans_b, int32_t M, int32_t N, int32_t K, double alpha,
+                  const double* A, int32_t lda, const double* B, int32_t ldb, double beta, double* C,
+                  int32_t ldc, void* stream);
 
 /* ---- BF16 matrix-core kernels of the learned initial guess --------------
    No reference counterpart: a pair-feature MLP that predicts the initial SCF

@@ -32,6 +32,7 @@
 
 #include "qbx_common.h"
 #include "qbx_eri_api.h"
+#include "qbx_linalg.h"
 #include "gen/eri_table.inc"
 
 using namespace qbx;
@@ -100,6 +101,7 @@ struct qbx_ctx {
     bool xcd_local = false;        // per-XCD J/K copies (set by the probe at init)
     int n_xcd = 1;
     int lds_max = 65536;           // LDS bytes per block on this device
+    DevBuf<double> la_X, la_F, la_P, la_Q, la_T, la_aux;   // density purification
     std::mutex mu;
 };
 
@@ -1360,6 +1362,84 @@ int qbx_jk(qbx_ctx* ctx, qbx_basis* b, int32_t n_set, const double* DJ, const do
         QBX_HIP(hipMemcpyAsync(b->d_D.p + NN * n_set, DK, NN * n_set * sizeof(double), hipMemcpyHostToDevice, st));
         jk_device(ctx, b, n_set, b->d_D.p, b->d_G.p, thr, mode, st);
         QBX_HIP(hipMemcpyAsync(G, b->d_G.p, NN * n_set * sizeof(double), hipMemcpyDeviceToHost, st));
+        QBX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int qbx_density_purify(qbx_ctx* ctx, int32_t n, int32_t nocc, const double* X, const double* F, double* D,
+                       int32_t max_iter, int32_t* iterations, double* idempotency)
+{
+    return guarded([&] {
+        if (!ctx) throw QbxError(QBX_EINVAL, "null context");
+        if (n <= 0 || nocc < 0 || nocc > n) throw QbxError(QBX_EINVAL, "need 0 <= nocc <= n and n > 0");
+        if (!X || !F || !D) throw QbxError(QBX_EINVAL, "null matrix pointer");
+        if (max_iter <= 0) throw QbxError(QBX_EINVAL, "max_iter must be positive");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        QBX_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        const size_t nn = (size_t)n * n;
+        ctx->la_X.ensure(nn);
+        ctx->la_F.ensure(nn);
+        ctx->la_P.ensure(nn);
+        ctx->la_Q.ensure(nn);
+        ctx->la_T.ensure(nn);
+        ctx->la_aux.ensure(2 * (size_t)n + 2);
+        double* dX = ctx->la_X.p;
+        double* dF = ctx->la_F.p;
+        double* dP = ctx->la_P.p;
+        double* dQ = ctx->la_Q.p;
+        double* dT = ctx->la_T.p;
+        QBX_HIP(hipMemcpyAsync(dX, X, nn * sizeof(double), hipMemcpyHostToDevice, st));
+        QBX_HIP(hipMemcpyAsync(dF, F, nn * sizeof(double), hipMemcpyHostToDevice, st));
+        // Fock matrix in the orthonormal basis: Fo = X' F X (into dQ)
+        gemm_f64(false, false, n, n, n, 1.0, dF, n, dX, n, 0.0, nullptr, 0, dT, n, st);
+        gemm_f64(true, false, n, n, n, 1.0, dX, n, dT, n, 0.0, nullptr, 0, dQ, n, st);
+        // spectral interval from Gershgorin discs
+        std::vector<double> lohi(2 * (size_t)n);
+        gershgorin(dQ, n, ctx->la_aux.p, st);
+        QBX_HIP(hipMemcpyAsync(lohi.data(), ctx->la_aux.p, 2 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+        QBX_HIP(hipStreamSynchronize(st));
+        double lo = lohi[0], hi = lohi[1];
+        for (int i = 1; i < n; ++i) {
+            lo = std::min(lo, lohi[2 * i]);
+            hi = std::max(hi, lohi[2 * i + 1]);
+        }
+        if (!(hi > lo)) hi = lo + 1.0;
+        sp2_start(dQ, n, lo, hi, dP, st);
+        // trace-correcting purification (SP2): P <- P^2 or 2P - P^2, whichever
+        // moves trace(P) toward nocc; quadratic convergence to the projector
+        // on the nocc lowest eigenvectors once the trace is right
+        double tf[2];
+        double* dtf = ctx->la_aux.p + 2 * (size_t)n;
+        auto measure = [&] {
+            trace_and_frobenius(dP, n, dtf, st);
+            QBX_HIP(hipMemcpyAsync(tf, dtf, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+            QBX_HIP(hipStreamSynchronize(st));
+        };
+        measure();
+        int it = 0, tail = -1;
+        const double N = (double)nocc;
+        for (; it < max_iter; ++it) {
+            const double tr = tf[0], fr = tf[1];
+            const double err = std::fabs(tr - fr);
+            if (tail < 0 && err < 1e-8 * std::max(1.0, N) && std::fabs(tr - N) < 1e-6) tail = 2;
+            if (tail == 0) break;
+            if (tail > 0) --tail;
+            if (std::fabs(fr - N) <= std::fabs(2.0 * tr - fr - N))
+                gemm_f64(false, false, n, n, n, 1.0, dP, n, dP, n, 0.0, nullptr, 0, dQ, n, st);
+            else
+                gemm_f64(false, false, n, n, n, -1.0, dP, n, dP, n, 2.0, dP, n, dQ, n, st);
+            std::swap(dP, dQ);
+            measure();
+        }
+        if (iterations) *iterations = it;
+        if (idempotency) *idempotency = std::fabs(tf[0] - tf[1]);
+        if (tail != 0 || !(std::fabs(tf[0] - N) < 1e-6))
+            throw QbxError(QBX_ENOCONV, "purification did not converge (no gap at the Fermi level, or max_iter too small)");
+        // D = X P X'
+        gemm_f64(false, false, n, n, n, 1.0, dX, n, dP, n, 0.0, nullptr, 0, dT, n, st);
+        gemm_f64(false, true, n, n, n, 1.0, dT, n, dX, n, 0.0, nullptr, 0, dQ, n, st);
+        QBX_HIP(hipMemcpyAsync(D, dQ, nn * sizeof(double), hipMemcpyDeviceToHost, st));
         QBX_HIP(hipStreamSynchronize(st));
     });
 }

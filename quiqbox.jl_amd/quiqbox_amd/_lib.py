@@ -46,6 +46,8 @@ SIGNATURES = {
     "qbx_set_density_screening": (_i32, [_vp, _d]),
     "qbx_basis_shells": (_i32, [_vp, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd]),
     "qbx_rhf_gradient": (_i32, [_vp, _vp, _pd, _pd, _i32, _pd, _pd, _d, _pd, _pd, _pd]),
+    "qbx_density_purify": (_i32, [_vp, _i32, _i32, _pd, _pd, _pd, _i32, _pi, _pd]),
+    "qbx_gemm_f64": (_i32, [_i32, _i32, _i32, _i32, _i32, _d, _vp, _i32, _vp, _i32, _d, _vp, _i32, _vp]),
     "qbx_gemm_bf16": (_i32, [_i32, _i32, _i32, _i32, _i32, ctypes.c_float, _vp, _i32, _vp, _i32, ctypes.c_float,
                              _vp, _i32, _i32, _vp, _vp, _vp]),
     "qbx_mlp_elementwise": (_i32, [_i32, _i64, _vp, _vp, _vp, _vp, _vp]),
@@ -72,6 +74,14 @@ def load(path=LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise RuntimeError("libqbx_hip.so is not built (expected at %s); run __graft_entry__.build()" % path)
+    # One HIP runtime per process.  torch (device buffers of the learned guess)
+    # ships its own libamdhip64 and makes its symbols global when imported; a
+    # library loaded after that binds to the same runtime.  Loaded the other
+    # way round, the process holds two runtimes and torch finds no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(L, name)

@@ -48,6 +48,26 @@ class Context:
     def set_density_screening(self, thr):
         _lib.check(_lib.load().qbx_set_density_screening(self.h, float(thr)))
 
+    def purified_density(self, X, F, nocc, max_iter=100):
+        """D = C[:, :nocc] C[:, :nocc]^T of eigen(X' F X) (C = X C') by
+        trace-correcting purification on the GPU (qbx_density_purify).
+        Returns (D, products taken, final |tr P - tr P^2|); raises QbxError
+        when it does not settle (no gap at the Fermi level)."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        n = X.shape[0]
+        if X.shape != (n, n) or F.shape != (n, n):
+            raise ValueError("X and F must be square matrices of one size")
+        D = np.empty((n, n), dtype=np.float64)
+        it = ctypes.c_int32()
+        idem = ctypes.c_double()
+        pd = ctypes.POINTER(ctypes.c_double)
+        with self.lock:
+            _lib.check(_lib.load().qbx_density_purify(self.h, n, int(nocc), X.ctypes.data_as(pd),
+                                                      F.ctypes.data_as(pd), D.ctypes.data_as(pd), int(max_iter),
+                                                      ctypes.byref(it), ctypes.byref(idem)))
+        return D, int(it.value), float(idem.value)
+
 
 def default_context():
     global _default_ctx

@@ -107,6 +107,7 @@ class DirectJK(TwoBodyOperator):
         self.switch_dE = switch_dE
         self.n_builds = 0
         self.n_mixed = 0
+        self.builds = []      # (full build?, precision, seconds) of every build
         self._ref = None      # (DJ, DK, G) of the last build
         self._since_full = 0
 
@@ -116,15 +117,17 @@ class DirectJK(TwoBodyOperator):
             self.precision = _lib.QBX_PREC_FP64
             self._ref = None   # next build is a full FP64 rebuild
 
-    def _build(self, DJ, DK, density_thr):
+    def _build(self, DJ, DK, density_thr, precision):
         ctx = self.basis.ctx
+        t0 = time.perf_counter()
         ctx.set_density_screening(density_thr)
         try:
-            G = self.basis.jk(DJ, DK, self.threshold, self.precision)
+            G = self.basis.jk(DJ, DK, self.threshold, precision)
         finally:
             ctx.set_density_screening(0.0)
+        self.builds.append((density_thr == 0.0, precision, time.perf_counter() - t0))
         self.n_builds += 1
-        if self.precision == _lib.QBX_PREC_MIXED:
+        if precision == _lib.QBX_PREC_MIXED:
             self.n_mixed += 1
         return G
 
@@ -134,10 +137,10 @@ class DirectJK(TwoBodyOperator):
         ref = self._ref
         if (self.incremental and ref is not None and ref[0].shape == DJ.shape
                 and self._since_full < self.rebuild_every):
-            G = ref[2] + self._build(DJ - ref[0], DK - ref[1], self.threshold)
+            G = ref[2] + self._build(DJ - ref[0], DK - ref[1], self.threshold, self.precision)
             self._since_full += 1
         else:
-            G = self._build(DJ, DK, 0.0)
+            G = self._build(DJ, DK, 0.0, self.precision)
             self._since_full = 0
         if self.incremental:
             self._ref = (DJ, DK, G)
@@ -235,10 +238,63 @@ def getEhfMO(HcoreMO, HeeIMO, Ns, Jab=None):
     return res + Jab[:Ns[0], :Ns[1]].sum()
 
 
+class LazyCoeff:
+    """Orbital coefficients getC(X, F), computed on first use: the SCF steps
+    whose density comes from purification keep the Fock matrix instead."""
+    __slots__ = ("X", "F", "_C")
+
+    def __init__(self, X, F):
+        self.X, self.F, self._C = X, F, None
+
+    def value(self):
+        if self._C is None:
+            self._C = getC(self.X, self.F)
+        return self._C
+
+    def __array__(self, dtype=None, copy=None):
+        C = self.value()
+        return C if dtype is None else C.astype(dtype)
+
+
+def _coeff(c):
+    return c.value() if isinstance(c, LazyCoeff) else c
+
+
+class PurifiedDensity:
+    """Density solver for the SCF steps: D = C[:, :N] C[:, :N]^T of eigen(X' F X)
+    by trace-correcting purification on the GPU (Context.purified_density),
+    with the diagonalization of getC/getD where purification does not settle
+    (no gap at the Fermi level)."""
+
+    def __init__(self, ctx, max_iter=100):
+        self.ctx = ctx
+        self.max_iter = max_iter
+        self.calls = 0
+        self.fallbacks = 0
+        self.products = 0
+
+    def __call__(self, X, F, N):
+        self.calls += 1
+        try:
+            D, it, _ = self.ctx.purified_density(X, F, N, self.max_iter)
+        except _lib.QbxError:
+            self.fallbacks += 1
+            return getD(getC(X, F), N)
+        self.products += it
+        return 0.5 * (D + D.T)
+
+
 def getCDFE(Hcore, twoBody, X, Ns, F):
-    """One SCF step: F -> C -> D -> F -> E (HartreeFock.jl:392-403)."""
-    Cn = tuple(getC(X, f) for f in F)
-    Dn = tuple(getD(c, n) for c, n in zip(Cn, Ns))
+    """One SCF step: F -> C -> D -> F -> E (HartreeFock.jl:392-403).  With a
+    density solver on the two-body operator (runHartreeFock's
+    densitySolver="purify") D comes from it and C is formed only if used."""
+    solver = getattr(twoBody, "density_solver", None)
+    if solver is None:
+        Cn = tuple(getC(X, f) for f in F)
+        Dn = tuple(getD(c, n) for c, n in zip(Cn, Ns))
+    else:
+        Cn = tuple(LazyCoeff(X, f) for f in F)
+        Dn = tuple(solver(X, f, n) for f, n in zip(F, Ns))
     Fn = getF(Hcore, twoBody, Dn)
     En = tuple(getE(Hcore, f, d) for f, d in zip(Fn, Dn))
     Dt = get2SpinQuantity(Dn)
@@ -931,13 +987,18 @@ def initializeHartreeFock(initCfg, basis, system, twoBody=None, schwarz=1e-12, p
 
 def runHartreeFock(nucInfo, basis, config=None, *, printInfo=True, infoLevel=defaultHFinfoLevel,
                    twoBody=None, schwarzThreshold=1e-12, precision=_lib.QBX_PREC_FP64, incremental=False,
-                   log=None):
+                   densitySolver="eigen", log=None):
     """runHartreeFock(nucInfo, basis, config) or runHartreeFock((spin, nucInfo), basis, config)
     (HartreeFock.jl:953-1046).  `basis` is a list of orbitals, a FlatBasis or a
     GPUBasis.  Returns HFfinalInfo with energy = (E_HF electronic, E_nuclear).
     Beyond the reference: `precision` (QBX_PREC_FP64 / QBX_PREC_MIXED) and
     `incremental` (density-screened incremental Fock builds) select the GPU
-    Fock-build mode; the defaults reproduce the reference's full FP64 builds."""
+    Fock-build mode; `densitySolver` "purify" forms each step's density by
+    purification on the GPU instead of diagonalization (the same matrix; the
+    coefficients of the result still come from one diagonalization).  The
+    defaults reproduce the reference's full FP64 builds and eigen steps."""
+    if densitySolver not in ("eigen", "purify"):
+        raise ValueError("densitySolver must be 'eigen' or 'purify'")
     t0 = time.time()
     if isinstance(nucInfo, tuple) and len(nucInfo) == 2 and isinstance(nucInfo[1], NuclearCluster):
         spin, nuc = nucInfo
@@ -964,6 +1025,11 @@ def runHartreeFock(nucInfo, basis, config=None, *, printInfo=True, infoLevel=def
         raise ValueError("The basis-set size should be no less than %d." % max(spin.layout))
     hamil, C0, gb = initializeHartreeFock(config.initial, basis, system, twoBody, schwarzThreshold, precision,
                                           incremental)
+    solver = None
+    if densitySolver == "purify":
+        if not isinstance(hamil.twoBody, DirectJK):
+            raise ValueError("densitySolver='purify' runs with the GPU Fock build")
+        solver = hamil.twoBody.density_solver = PurifiedDensity(gb.ctx)
     t1 = time.time()
     if printInfo and infoLevel > 1:
         (log or print)("Hartree–Fock (HF) Initialization:\n•HF Type: %s\n•Basis Set Size: %d\n"
@@ -976,10 +1042,16 @@ def runHartreeFock(nucInfo, basis, config=None, *, printInfo=True, infoLevel=def
     occu = getOrbitalOccupations(HFT, hamil.orthonormalization, Ns, Fs)
     Enuc = nucRepulsion(nuc)
     Eel = vars_[0].shared.Etots[-1]
-    res = HFfinalInfo(system, (Eel, Enuc), tuple(tv.Cs[-1] for tv in vars_), tuple(tv.Ds[-1] for tv in vars_),
-                      Fs, occu, vars_, converged, gb,
-                      dict(init_s=t1 - t0, scf_s=t2 - t1, fock_builds=getattr(hamil.twoBody, "n_builds", None),
-                           mixed_builds=getattr(hamil.twoBody, "n_mixed", None)))
+    Cfin = tuple(_coeff(tv.Cs[-1]) for tv in vars_)
+    timing = dict(init_s=t1 - t0, scf_s=t2 - t1, fock_builds=getattr(hamil.twoBody, "n_builds", None),
+                  mixed_builds=getattr(hamil.twoBody, "n_mixed", None),
+                  builds=list(getattr(hamil.twoBody, "builds", [])))
+    if solver is not None:
+        hamil.twoBody.density_solver = None
+        timing.update(purified_steps=solver.calls - solver.fallbacks, purify_fallbacks=solver.fallbacks,
+                      purify_products=solver.products)
+    res = HFfinalInfo(system, (Eel, Enuc), Cfin, tuple(tv.Ds[-1] for tv in vars_),
+                      Fs, occu, vars_, converged, gb, timing)
     if printInfo:
         (log or print)("Hartree–Fock Energy ¦ Nuclear Repulsion ¦ Total Energy\n%.10f Ha   %.10f Ha   %.10f Ha"
                        % (Eel, Enuc, Eel + Enuc))
