@@ -160,20 +160,26 @@ def cpu_baseline(flat, n_quartets_total, seconds=12.0, threads=None):
                 fock_build_s_extrapolated=n_quartets_total / rate)
 
 
-def run_scf(nuc, gb, precision, thr=1e-8, incremental=False):
+def run_scf(nuc, gb, precision, thr=1e-8, incremental=False, density_solver="eigen"):
     import quiqbox_amd as qb
     cfg = qb.HFconfig(initial="CoreH", strategy=qb.SCFconfig(("ADIIS", "DIIS"), (1e-4, thr)), maxStep=120)
     t0 = time.perf_counter()
     res = qb.runHartreeFock(nuc, gb, cfg, printInfo=False, schwarzThreshold=1e-12, precision=precision,
-                            incremental=incremental)
+                            incremental=incremental, densitySolver=density_solver)
     return res, time.perf_counter() - t0
+
+
+def build_log(res):
+    """One token per Fock build: F/i (full/incremental), d/m (FP64/mixed), milliseconds."""
+    return " ".join("%s%s:%.0f" % ("F" if f else "i", "m" if p else "d", 1e3 * t) for f, p, t in res.timing["builds"])
 
 
 def learned_section(nuc, gb, res64, t64, n_train, epochs):
     """Learned initial guess: train the pair-feature network on GPU-converged
     densities of small water clusters (cc-pVDZ, 2-6 molecules, seeds disjoint
     from the evaluation system), then start the 600-function water SCF from
-    it, in FP64 and in mixed precision with incremental builds."""
+    it, in FP64 and in mixed precision with incremental builds and purified
+    densities."""
     import quiqbox_amd as qb
     from quiqbox_amd import learned_guess as lg
     from quiqbox_amd.systems import water_box
@@ -188,7 +194,8 @@ def learned_section(nuc, gb, res64, t64, n_train, epochs):
     strat = qb.SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-8))
     out = {"training_systems": len(data), "training_pairs": int(sum(len(x) for x, _ in data)),
            "data_s": t_data, "train_s": t_train, "epochs": epochs, "loss_first": hist[0], "loss_last": hist[-1]}
-    for tag, prec, inc in (("fp64", qb.QBX_PREC_FP64, False), ("fast", qb.QBX_PREC_MIXED, True)):
+    for tag, prec, inc, ds in (("fp64", qb.QBX_PREC_FP64, False, "eigen"),
+                               ("fast", qb.QBX_PREC_MIXED, True, "purify")):
         t0 = time.perf_counter()
         info = {}
         cfg = lg.guess_config(model, nuc, gb, strat, 120, info=info)
@@ -198,10 +205,11 @@ def learned_section(nuc, gb, res64, t64, n_train, epochs):
         out["electrons_predicted"] = info["electrons_predicted"]
         out["learned_used"] = info["learned"]
         res = qb.runHartreeFock(nuc, gb, cfg, printInfo=False, schwarzThreshold=1e-12, precision=prec,
-                                incremental=inc)
+                                incremental=inc, densitySolver=ds)
         t = time.perf_counter() - t0
         log("RHF %s from the learned guess: converged=%s in %.2f s, %d Fock builds, dE vs FP64 %.2e" % (
             tag, res.converged, t, res.timing["fock_builds"], res.energy[0] - res64.energy[0]))
+        log("  builds: " + build_log(res))
         out[tag + "_wall_s"] = t
         out[tag + "_fock_builds"] = res.timing["fock_builds"]
         out[tag + "_converged"] = bool(res.converged)
@@ -274,18 +282,25 @@ def rhf_section(hip, cpu_seconds, no_cpu):
     log("RHF FP64: E=%.10f converged=%s in %.2f s, %d Fock builds" % (
         res64.energy[0] + res64.energy[1], res64.converged, t64, res64.timing["fock_builds"]))
     rhf_section.last = (nuc, gb, res64, t64)
-    resmx, tmx = run_scf(nuc, gb, qb.QBX_PREC_MIXED, incremental=True)
-    log("RHF mixed+incremental: E=%.10f converged=%s in %.2f s (%s of %s builds mixed)" % (
+    respu, tpu = run_scf(nuc, gb, qb.QBX_PREC_FP64, density_solver="purify")
+    log("RHF FP64 + purified densities: E=%.10f converged=%s in %.2f s, %d Fock builds" % (
+        respu.energy[0] + respu.energy[1], respu.converged, tpu, respu.timing["fock_builds"]))
+    resmx, tmx = run_scf(nuc, gb, qb.QBX_PREC_MIXED, incremental=True, density_solver="purify")
+    log("RHF mixed+incremental+purified: E=%.10f converged=%s in %.2f s (%s of %s builds mixed)" % (
         resmx.energy[0] + resmx.energy[1], resmx.converged, tmx, resmx.timing["mixed_builds"],
         resmx.timing["fock_builds"]))
+    log("  builds: " + build_log(resmx))
     out = {"workload": "24 H2O (liquid density, seeded orientations), cc-pVDZ, %d Cartesian functions, "
                        "core-Hamiltonian guess, ADIIS then DIIS, |dE| < 1e-8 Eh, Schwarz 1e-12" % n,
            "fock_build_ms": 1e3 * t_build, "fock_build_quartets_per_s": n_cart / t_build,
            "screened_cart_quartets": n_cart,
            "fp64_wall_s": t64, "fp64_energy": res64.energy[0] + res64.energy[1],
            "fp64_converged": bool(res64.converged), "fp64_fock_builds": res64.timing["fock_builds"],
+           "fp64_purify_wall_s": tpu, "fp64_purify_fock_builds": respu.timing["fock_builds"],
+           "fp64_purify_energy_error_vs_fp64": (respu.energy[0] - res64.energy[0]),
            "fast_wall_s": tmx, "fast_mode": "mixed precision (FP32 below Schwarz 1e-5, FP64 rebuilds near "
-                                           "convergence) + density-screened incremental builds",
+                                           "convergence) + density-screened incremental builds + "
+                                           "densities by purification on the GPU",
            "fast_energy_error_vs_fp64": (resmx.energy[0] - res64.energy[0]),
            "fast_converged": bool(resmx.converged), "fast_fock_builds": resmx.timing["fock_builds"],
            "fast_mixed_builds": resmx.timing["mixed_builds"]}

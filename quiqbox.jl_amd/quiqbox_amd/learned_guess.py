@@ -70,52 +70,66 @@ def atom_blocks(nuc, gb):
     return blocks
 
 
+def _padded_index(blocks, n):
+    """(n_atoms, NMAX) function indices of every atom, padded with n (a zero
+    row/column appended to the matrices)."""
+    idx = np.full((len(blocks), NMAX), n, dtype=np.int64)
+    for a, b in enumerate(blocks):
+        idx[a, :len(b)] = b
+    return idx
+
+
+def _ext(M):
+    n = M.shape[0]
+    E = np.zeros((n + 1, n + 1))
+    E[:n, :n] = M
+    return E
+
+
 def pair_features(nuc, blocks, S, H):
     """Raw features (n_atoms^2 x N_FEAT) of all ordered atom pairs.  H is
     (T, Vn): the kinetic matrix and the attraction matrix of every nucleus."""
     T, Vn = H
     na = len(blocks)
-    R = nuc.coord_array
-    X = np.zeros((na * na, N_FEAT))
+    n = S.shape[0]
+    idx = _padded_index(blocks, n)
+    I, J = idx[:, None, :, None], idx[None, :, None, :]
+    Sb = _ext(S)[I, J]                                  # (na, na, NMAX, NMAX)
+    Hb = _ext(T)[I, J]
     for a in range(na):
-        for b in range(na):
-            r = a * na + b
-            ia, ib = blocks[a], blocks[b]
-            s = np.zeros((NMAX, NMAX))
-            h = np.zeros((NMAX, NMAX))
-            s[:len(ia), :len(ib)] = S[np.ix_(ia, ib)]
-            hl = T[np.ix_(ia, ib)] + Vn[a][np.ix_(ia, ib)]
-            if b != a:
-                hl = hl + Vn[b][np.ix_(ia, ib)]
-            h[:len(ia), :len(ib)] = hl
-            X[r, :NMAX * NMAX] = s.ravel()
-            X[r, NMAX * NMAX:2 * NMAX * NMAX] = h.ravel()
-            o = 2 * NMAX * NMAX
-            X[r, o + ELEMENTS.index(nuc.symbols[a])] = 1.0
-            X[r, o + len(ELEMENTS) + ELEMENTS.index(nuc.symbols[b])] = 1.0
-            d = float(np.linalg.norm(R[a] - R[b]))
-            X[r, o + 2 * len(ELEMENTS):] = (1.0 if a == b else 0.0, d, np.exp(-d), 1.0 / (1.0 + d))
+        Va = _ext(Vn[a])
+        Hb[a] += Va[idx[a][None, :, None], idx[:, None, :]]       # V(A)_AB for every B
+        Vab = Va[idx[:, :, None], idx[a][None, None, :]]          # V(A)_BA for every B
+        Vab[a] = 0.0                                               # the pair (A, A) holds V(A) once
+        Hb[:, a] += Vab
+    X = np.zeros((na * na, N_FEAT))
+    X[:, :NMAX * NMAX] = Sb.reshape(na * na, -1)
+    X[:, NMAX * NMAX:2 * NMAX * NMAX] = Hb.reshape(na * na, -1)
+    o = 2 * NMAX * NMAX
+    el = np.array([ELEMENTS.index(sym) for sym in nuc.symbols])
+    X[np.arange(na * na), o + np.repeat(el, na)] = 1.0
+    X[np.arange(na * na), o + len(ELEMENTS) + np.tile(el, na)] = 1.0
+    R = nuc.coord_array
+    d = np.sqrt(((R[:, None, :] - R[None, :, :]) ** 2).sum(-1)).ravel()
+    X[:, o + 2 * len(ELEMENTS)] = np.eye(na).ravel()
+    X[:, o + 2 * len(ELEMENTS) + 1] = d
+    X[:, o + 2 * len(ELEMENTS) + 2] = np.exp(-d)
+    X[:, o + 2 * len(ELEMENTS) + 3] = 1.0 / (1.0 + d)
     return X
 
 
 def pair_targets(blocks, D):
     na = len(blocks)
-    Y = np.zeros((na * na, N_OUT))
-    for a in range(na):
-        for b in range(na):
-            t = np.zeros((NMAX, NMAX))
-            t[:len(blocks[a]), :len(blocks[b])] = D[np.ix_(blocks[a], blocks[b])]
-            Y[a * na + b] = t.ravel()
-    return Y
+    idx = _padded_index(blocks, D.shape[0])
+    return _ext(D)[idx[:, None, :, None], idx[None, :, None, :]].reshape(na * na, N_OUT)
 
 
 def assemble_density(blocks, Y, n):
     na = len(blocks)
-    D = np.zeros((n, n))
-    for a in range(na):
-        for b in range(na):
-            t = Y[a * na + b].reshape(NMAX, NMAX)
-            D[np.ix_(blocks[a], blocks[b])] = t[:len(blocks[a]), :len(blocks[b])]
+    idx = _padded_index(blocks, n)
+    D = np.zeros((n + 1, n + 1))
+    D[idx[:, None, :, None], idx[None, :, None, :]] = Y.reshape(na, na, NMAX, NMAX)
+    D = D[:n, :n]
     return 0.5 * (D + D.T)
 
 
