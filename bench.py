@@ -92,6 +92,18 @@ def copy_calibration(n_bytes=1 << 30, reps=3):
                                            "kernel": "elementwise neg (float64)"}}), flush=True)
 
 
+def kernel_from_profile(tag):
+    """Symbol of the class-`tag` FP64 J/K kernel in the committed PMC summary."""
+    path = os.path.join(ROOT, "profiles", "eri_jk64_hbm_traffic_592bf.txt")
+    if os.path.exists(path):
+        with open(path) as f:
+            for line in f:
+                w = line.split()
+                if len(w) >= 3 and w[0] == "kernel" and w[1] == tag:
+                    return w[2]
+    return "J/K kernel of class (%s), FP64" % tag
+
+
 def traffic_from_profile(tag):
     """HBM bytes per launch of the class-`tag` JK kernel from the committed
     PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy calibration), or
@@ -420,7 +432,7 @@ def main():
                    "screened_shell_quartets": int(stats["shell_quartets"].sum()),
                    "screened_cart_quartets": n_cart,
                    "primitive_shell_quartets": int(stats["prim_quartets"].sum())},
-        "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "eri_kernel<%s, double, JK>" % tag,
+        "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": kernel_from_profile(tag),
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profile(tag),
                      "kernel_ms": float(per_class[dom]), "all_eri_kernels_tflops": total_tflops},
