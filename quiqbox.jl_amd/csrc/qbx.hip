@@ -75,7 +75,10 @@ template <typename T> struct DevBuf {
 };
 
 constexpr double kPi = 3.14159265358979323846;
-constexpr int kStreams = 4;
+// auxiliary streams of a Fock build: a process gets 4 hardware queues, one of
+// them the context's main stream, so a fourth auxiliary stream would share a
+// queue with another and leave its work serialized behind it
+constexpr int kStreams = 3;
 // wave slots of a staged class's workspace (persistent grid): 8 per CU
 constexpr int kStageSlots = 2048;
 // primitive pairs with |c_a c_b exp(-ab/p R^2) / p| below this are dropped.
@@ -874,7 +877,8 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
         pdmax = b->d_pdmax.p;
     }
     // Without timing, the class kernels run concurrently on the auxiliary
-    // streams (largest work first); they only meet in the J/K atomics.
+    // streams (staged classes first, then largest work first); they only meet
+    // in the J/K atomics.
     const bool fork = !ctx->timing;
     if (fork) {
         QBX_HIP(hipEventRecord(ctx->ev_fork, st));
@@ -883,6 +887,10 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
     std::vector<int> order(QBX_N_CLASSES);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        // staged classes first: few long-running waves each, they overlap with
+        // the wide low-L kernels instead of running alone at the end
+        const bool sx = qbx_class_ws[x] > 0, sy = qbx_class_ws[y] > 0;
+        if (sx != sy) return sx;
         return b->t64[x].total + b->t32[x].total > b->t64[y].total + b->t32[y].total;
     });
     for (int oi = 0; oi < QBX_N_CLASSES; ++oi) {
