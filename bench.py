@@ -207,7 +207,8 @@ def learned_section(nuc, gb, res64, t64, n_train, epochs):
     out = {"training_systems": len(data), "training_pairs": int(sum(len(x) for x, _ in data)),
            "data_s": t_data, "train_s": t_train, "epochs": epochs, "loss_first": hist[0], "loss_last": hist[-1]}
     for tag, prec, inc, ds in (("fp64", qb.QBX_PREC_FP64, False, "eigen"),
-                               ("fast", qb.QBX_PREC_MIXED, True, "purify")):
+                               ("fast", qb.QBX_PREC_MIXED, True, "purify"),
+                               ("fast_fp64", qb.QBX_PREC_FP64, True, "purify")):
         t0 = time.perf_counter()
         info = {}
         cfg = lg.guess_config(model, nuc, gb, strat, 120, info=info)
@@ -227,6 +228,7 @@ def learned_section(nuc, gb, res64, t64, n_train, epochs):
         out[tag + "_converged"] = bool(res.converged)
         out[tag + "_energy_error_vs_fp64"] = res.energy[0] - res64.energy[0]
     out["speedup_fast_learned_vs_fp64_core"] = t64 / out["fast_wall_s"]
+    out["speedup_fast_fp64_learned_vs_fp64_core"] = t64 / out["fast_fp64_wall_s"]
     return out
 
 
@@ -340,7 +342,7 @@ def main():
     ap.add_argument("--learned-systems", type=int, default=40,
                     help="training systems of the learned-guess section (0: skip it)")
     ap.add_argument("--learned-epochs", type=int, default=300)
-    ap.add_argument("--grad-steps", type=int, default=30,
+    ap.add_argument("--grad-steps", type=int, default=100,
                     help="basis-optimization steps of the gradient section (0: skip it)")
     ap.add_argument("--serial", action="store_true",
                     help="run the class kernels of each build one after another on one stream (for profiling)")
