@@ -62,6 +62,12 @@ GRAD_LMAX = 1
 # classes evaluated in groups of bra components through a workspace
 STAGED = {(2, 1, 1, 1), (2, 1, 2, 1), (2, 2, 1, 1), (2, 2, 2, 0), (2, 2, 2, 1), (2, 2, 2, 2)}
 STAGE_ACC = 32     # accumulators per group
+# Classes whose staged kernel runs at one wave per SIMD with larger groups: the
+# register file of a lone wave (VGPRs plus AGPRs as spill space) holds 48
+# accumulators, so the class makes 21 passes over its primitive quartets
+# instead of 31.  Measured on the 592-function cluster: (dd|dd) 4.1 -> 2.7 ms;
+# the other staged classes lose more latency hiding than they save.
+STAGE_ONE_WAVE = {(2, 2, 2, 2): 48}
 
 
 def cart(L):
@@ -193,12 +199,14 @@ def gen_staged(la, lb, lc, ld, es, fs, eidx, fidx, C_, D_):
     nE, nF = len(es), len(fs)
     nC, nD = len(C_), len(D_)
     NCD = nC * nD
-    G = -(-nE * nF // STAGE_ACC)
+    acc_per_group = STAGE_ONE_WAVE.get((la, lb, lc, ld), STAGE_ACC)
+    G = -(-nE * nF // acc_per_group)
     G = min(G, nE)
     bounds = [k * nE // G for k in range(G + 1)]
     code = []
     flops_vrr_exec = 0
     code.append("    static constexpr bool STAGED = true;")
+    code.append("    static constexpr int WAVES_PER_SIMD = %d;" % (1 if (la, lb, lc, ld) in STAGE_ONE_WAVE else 2))
     code.append("    static constexpr int NGRP = %d;" % G)
     code.append("    static constexpr int NCD = %d;" % NCD)
     code.append("    // group g holds bra components e in [ge0(g), ge0(g + 1))")

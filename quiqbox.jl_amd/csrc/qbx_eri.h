@@ -858,7 +858,7 @@ __device__ __forceinline__ void staged_chunk(const EriArgs& a, int bi, int chunk
 }
 
 template <int LA, int LB, int LC, int LD, typename R, int MODE>
-__global__ __launch_bounds__(64 * STAGE_WAVES_PER_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
+__global__ __launch_bounds__(64 * STAGE_WAVES_PER_BLOCK) __attribute__((amdgpu_waves_per_eu(EriClass<LA, LB, LC, LD, R>::WAVES_PER_SIMD, EriClass<LA, LB, LC, LD, R>::WAVES_PER_SIMD)))
 void eri_staged_kernel(EriArgs a)
 {
     using Cls = EriClass<LA, LB, LC, LD, R>;
@@ -893,7 +893,7 @@ void eri_staged_kernel(EriArgs a)
 // the task's chunks, K goes to the rows, and the rows are flushed to K once
 // per task.
 template <int LA, int LB, int LC, int LD, typename R>
-__global__ __launch_bounds__(64 * STAGE_WAVES_PER_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
+__global__ __launch_bounds__(64 * STAGE_WAVES_PER_BLOCK) __attribute__((amdgpu_waves_per_eu(EriClass<LA, LB, LC, LD, R>::WAVES_PER_SIMD, EriClass<LA, LB, LC, LD, R>::WAVES_PER_SIMD)))
 void eri_staged_rows_kernel(EriArgs a)
 {
     using Cls = EriClass<LA, LB, LC, LD, R>;
