@@ -68,6 +68,7 @@ STAGE_ACC = 32     # accumulators per group
 # instead of 31.  Measured on the 592-function cluster: (dd|dd) 4.1 -> 2.7 ms;
 # the other staged classes lose more latency hiding than they save.
 STAGE_ONE_WAVE = {(2, 2, 2, 2): 48}
+TWIN_MAX_ACC = 9   # must match qbx_eri_api.h
 
 
 def cart(L):
@@ -670,8 +671,10 @@ def main(outdir):
     os.makedirs(outdir, exist_ok=True)
     classes, pcl = class_list()
     flops = {}
+    twin_ok = []
     for (la, lb, lc, ld) in classes:
         code, fv, fh, nacc, fexec = gen_class(la, lb, lc, ld)
+        twin_ok.append(fexec is None and nacc <= TWIN_MAX_ACC)
         tag = "%d%d%d%d" % (la, lb, lc, ld)
         flops[tag] = {"vrr_per_prim_quartet": fv, "hrr_per_quartet": fh, "n_acc": nacc,
                       "staged": fexec is not None}
@@ -721,6 +724,9 @@ def main(outdir):
              "// workspace values per lane of the staged classes (0: not staged)",
              "static const int qbx_class_ws[%d] = {%s};" % (
                  len(classes), ", ".join(str(stage_ws(*c)) for c in classes)),
+             "// classes with a twin-bra kernel (not staged, at most %d accumulators)" % TWIN_MAX_ACC,
+             "static const int qbx_class_twin[%d] = {%s};" % (
+                 len(classes), ", ".join("1" if t else "0" for t in twin_ok)),
              "#define QBX_FOR_EACH_CLASS(X) \\"]
     lines += ["    X(%d, %d, %d, %d) \\" % c for c in classes]
     lines.append("")

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <memory>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <stdexcept>
@@ -110,6 +111,7 @@ struct qbx_ctx {
     bool xcd_local = false;        // per-XCD J/K copies (set by the probe at init)
     int n_xcd = 1;
     int lds_max = 65536;           // LDS bytes per block on this device
+    bool twins = true;             // twin-bra groups in the J/K build (QBX_TWINS=0: off)
     DevBuf<double> la_X, la_F, la_P, la_Q, la_T, la_aux;   // density purification
     std::mutex mu;
 };
@@ -133,6 +135,14 @@ struct TaskSet {
     int64_t total = 0;
     int nwave = 0;
     int nbra = 0;
+};
+
+// Twin-bra groups of one ERI class and precision (see TWIN_MAX): task prefix
+// and, per group, TWIN_MAX records (bra, first ket, end of kets).
+struct TwinSet {
+    DevBuf<int> gpre;
+    DevBuf<int> gmem;
+    int ngroup = 0, ntask = 0, twmax = 0;
 };
 
 struct HostShell {
@@ -163,6 +173,13 @@ struct qbx_basis {
     double task_thr = -1.0, task_cut = -1.0;
     int task_mode = -1;
     TaskSet t64[QBX_N_CLASSES], t32[QBX_N_CLASSES];
+    // twin bras: groups of positions in each (Schwarz-sorted) pair class; for
+    // the J/K build of a class with groups, the other bras' task lists and
+    // the groups' own
+    std::vector<std::vector<int>> twins[QBX_N_PAIR_CLASSES];
+    bool use_twin[QBX_N_CLASSES] = {};
+    TaskSet s64[QBX_N_CLASSES], s32[QBX_N_CLASSES];
+    TwinSet w64[QBX_N_CLASSES];
     DevBuf<double> ws[QBX_N_CLASSES];   // staged-class workspaces
     // work buffers
     DevBuf<double> d_D, d_J, d_K, d_G;
@@ -639,6 +656,45 @@ void attach_workspace(qbx_basis* b, int cls, int nwave, EriArgs& a)
 }
 
 // Schwarz bounds, then every pair class sorted by descending bound.
+// Twin bras.  Shells that share centre, L and exponents form a family (the
+// segmented copies of a general contraction); shell pairs of one pair class
+// whose shells come from the same two families and that kept the same
+// primitive pairs have identical primitive geometry and differ only in the
+// c_a c_b factors.  Groups of up to TWIN_MAX such pairs, in Schwarz order.
+void find_twins(qbx_basis* b)
+{
+    const int ns = (int)b->shells.size();
+    std::map<std::vector<double>, int> fam_of;
+    std::vector<int> fam(ns);
+    for (int s = 0; s < ns; ++s) {
+        const HostShell& H = b->shells[s];
+        std::vector<double> key = {H.c[0], H.c[1], H.c[2], (double)H.L, (double)H.nprim};
+        for (int k = 0; k < H.nprim; ++k) key.push_back(b->pexp[H.poff + k]);
+        auto it = fam_of.find(key);
+        if (it == fam_of.end()) it = fam_of.emplace(key, (int)fam_of.size()).first;
+        fam[s] = it->second;
+    }
+    for (int c = 0; c < QBX_N_PAIR_CLASSES; ++c) {
+        const PairClass& P = b->pc[c];
+        std::map<std::vector<int>, std::vector<int>> sig;
+        for (int k = 0; k < P.n(); ++k) {
+            std::vector<int> key = {fam[P.shA[k]], fam[P.shB[k]], P.ppn[k]};
+            for (int u = 0; u < P.ppn[k]; ++u) key.push_back(b->ppk[P.ppo[k] + u]);
+            sig[key].push_back(k);
+        }
+        b->twins[c].clear();
+        for (auto& kv : sig) {
+            const std::vector<int>& m = kv.second;
+            for (size_t o = 0; o + 1 < m.size(); o += TWIN_MAX) {
+                const size_t e = std::min(m.size(), o + (size_t)TWIN_MAX);
+                if (e - o >= 2) b->twins[c].emplace_back(m.begin() + o, m.begin() + e);
+            }
+        }
+        std::sort(b->twins[c].begin(), b->twins[c].end(),
+                  [](const std::vector<int>& x, const std::vector<int>& y) { return x[0] < y[0]; });
+    }
+}
+
 void ensure_schwarz(qbx_basis* b)
 {
     if (b->schwarz) return;
@@ -701,6 +757,7 @@ void ensure_schwarz(qbx_basis* b)
     }
     QBX_HIP(hipStreamSynchronize(st));
     b->schwarz = true;
+    find_twins(b);
 }
 
 // number of leading entries of a descending array that are >= v
@@ -719,12 +776,29 @@ struct TaskCounts {
     std::vector<int> s64, s32;   // K-row blocks per bra, prefix sums
 };
 
-void task_counts(const qbx_basis* b, int cls, double thr, int mode, double cut, TaskCounts& tc)
+// Surviving kets of bra i: FP64 [0, c64), FP32 [c64, cnt) in the ket class's order.
+void bra_range(const qbx_basis* b, int cls, int i, double thr, int mode, double cut, int& c64, int& cnt)
 {
     const int X = class_pc(cls, 0), Y = class_pc(cls, 1);
     const PairClass& PX = b->pc[X];
     const PairClass& PY = b->pc[Y];
-    const int nx = PX.n();
+    const double qi = PX.Q[i];
+    if (thr <= 0.0) cnt = PY.n();
+    else cnt = qi > 0.0 ? count_ge(PY.Q, thr / qi) : 0;
+    if (X == Y) cnt = std::min(cnt, i + 1);
+    c64 = cnt;
+    if (mode == QBX_PREC_MIXED) {
+        const int kc = qi > 0.0 ? count_ge(PY.Q, cut / qi) : 0;
+        c64 = std::min(cnt, kc);
+    }
+}
+
+// start64: per bra, the first FP64 ket left to these lists (the twin groups
+// take the kets before it); null: all of them
+void task_counts(const qbx_basis* b, int cls, double thr, int mode, double cut, TaskCounts& tc,
+                 const std::vector<int>* start64 = nullptr)
+{
+    const int nx = b->pc[class_pc(cls, 0)].n();
     tc.p64.assign(nx + 1, 0);
     tc.p32.assign(nx + 1, 0);
     tc.k64.assign(std::max(nx, 1), 0);
@@ -734,25 +808,76 @@ void task_counts(const qbx_basis* b, int cls, double thr, int mode, double cut, 
     tc.s64.assign(nx + 1, 0);
     tc.s32.assign(nx + 1, 0);
     for (int i = 0; i < nx; ++i) {
-        const double qi = PX.Q[i];
-        int cnt;
-        if (thr <= 0.0) cnt = PY.n();
-        else cnt = qi > 0.0 ? count_ge(PY.Q, thr / qi) : 0;
-        if (X == Y) cnt = std::min(cnt, i + 1);
-        int c64 = cnt;
-        if (mode == QBX_PREC_MIXED) {
-            int kc = qi > 0.0 ? count_ge(PY.Q, cut / qi) : 0;
-            c64 = std::min(cnt, kc);
-        }
-        tc.k64[i] = 0;
+        int c64, cnt;
+        bra_range(b, cls, i, thr, mode, cut, c64, cnt);
+        const int k0 = start64 ? std::min((*start64)[i], c64) : 0;
+        tc.k64[i] = k0;
         tc.k32[i] = c64;
-        tc.p64[i + 1] = tc.p64[i] + c64;
+        tc.p64[i + 1] = tc.p64[i] + (c64 - k0);
         tc.p32[i + 1] = tc.p32[i] + (cnt - c64);
-        tc.w64[i + 1] = tc.w64[i] + (c64 + 63) / 64;
+        tc.w64[i + 1] = tc.w64[i] + (c64 - k0 + 63) / 64;
         tc.w32[i + 1] = tc.w32[i] + (cnt - c64 + 63) / 64;
-        tc.s64[i + 1] = tc.s64[i] + ((c64 + 63) / 64 + ROW_CHUNKS - 1) / ROW_CHUNKS;
+        tc.s64[i + 1] = tc.s64[i] + ((c64 - k0 + 63) / 64 + ROW_CHUNKS - 1) / ROW_CHUNKS;
         tc.s32[i + 1] = tc.s32[i] + ((cnt - c64 + 63) / 64 + ROW_CHUNKS - 1) / ROW_CHUNKS;
     }
+}
+
+void upload_tasks(TaskSet& t64, TaskSet& t32, const TaskCounts& tc, int nx, hipStream_t st)
+{
+    t64.prefix.upload(tc.p64, st);
+    t64.koff.upload(tc.k64, st);
+    t64.cpre.upload(tc.w64, st);
+    t64.spre.upload(tc.s64, st);
+    t64.nsuper = tc.s64[nx];
+    t64.total = tc.p64[nx];
+    t64.nwave = tc.w64[nx];
+    t64.nbra = nx;
+    t32.prefix.upload(tc.p32, st);
+    t32.koff.upload(tc.k32, st);
+    t32.cpre.upload(tc.w32, st);
+    t32.spre.upload(tc.s32, st);
+    t32.nsuper = tc.s32[nx];
+    t32.total = tc.p32[nx];
+    t32.nwave = tc.w32[nx];
+    t32.nbra = nx;
+}
+
+// Task lists of the twin groups of class cls: the FP64 kets that every member
+// of a group keeps, [0, common), in chunks of 64, ROW_CHUNKS chunks a task.
+// Each member's kets beyond the common prefix, and all FP32 kets, stay with
+// the per-bra lists (start64 receives the common prefix of every member), so
+// the shared evaluation never covers a ket that a member would not.
+void twin_tasks(const qbx_basis* b, int cls, double thr, int mode, double cut, TwinSet& w64,
+                std::vector<int>& start64, hipStream_t st)
+{
+    const std::vector<std::vector<int>>& groups = b->twins[class_pc(cls, 0)];
+    std::vector<int> pre(1, 0), mem;
+    int twmax = 0;
+    for (const std::vector<int>& m : groups) {
+        int common = INT32_MAX;
+        for (int i : m) {
+            int c64, cnt;
+            bra_range(b, cls, i, thr, mode, cut, c64, cnt);
+            common = std::min(common, c64);
+        }
+        if (common <= 0) continue;
+        twmax = std::max(twmax, (int)m.size());
+        for (size_t t = 0; t < (size_t)TWIN_MAX; ++t) {
+            const bool in = t < m.size();
+            mem.push_back(in ? m[t] : -1);
+            mem.push_back(0);
+            mem.push_back(in ? common : 0);
+            if (in) start64[m[t]] = common;
+        }
+        const int ch = (common + 63) / 64;
+        pre.push_back(pre.back() + (ch + ROW_CHUNKS - 1) / ROW_CHUNKS);
+    }
+    const int ng = (int)pre.size() - 1;
+    w64.gpre.upload(pre, st);
+    if (ng > 0) w64.gmem.upload(mem, st);
+    w64.ngroup = ng;
+    w64.ntask = pre.back();
+    w64.twmax = twmax;
 }
 
 void ensure_tasks(qbx_basis* b, double thr, int mode, double cut)
@@ -763,23 +888,25 @@ void ensure_tasks(qbx_basis* b, double thr, int mode, double cut)
     for (int cls = 0; cls < QBX_N_CLASSES; ++cls) {
         TaskCounts tc;
         task_counts(b, cls, thr, mode, cut, tc);
-        const int nx = b->pc[class_pc(cls, 0)].n();
-        b->t64[cls].prefix.upload(tc.p64, st);
-        b->t64[cls].koff.upload(tc.k64, st);
-        b->t64[cls].cpre.upload(tc.w64, st);
-        b->t64[cls].spre.upload(tc.s64, st);
-        b->t64[cls].nsuper = tc.s64[nx];
-        b->t64[cls].total = tc.p64[nx];
-        b->t64[cls].nwave = tc.w64[nx];
-        b->t64[cls].nbra = nx;
-        b->t32[cls].prefix.upload(tc.p32, st);
-        b->t32[cls].koff.upload(tc.k32, st);
-        b->t32[cls].cpre.upload(tc.w32, st);
-        b->t32[cls].spre.upload(tc.s32, st);
-        b->t32[cls].nsuper = tc.s32[nx];
-        b->t32[cls].total = tc.p32[nx];
-        b->t32[cls].nwave = tc.w32[nx];
-        b->t32[cls].nbra = nx;
+        const int X = class_pc(cls, 0);
+        const int nx = b->pc[X].n();
+        upload_tasks(b->t64[cls], b->t32[cls], tc, nx, st);
+        b->use_twin[cls] = false;
+        // only where bra and ket classes differ: within one pair class a bra's
+        // kets end at its own Schwarz-sorted position, so the members' common
+        // prefix is short and the group's fixed cost per block outweighs it
+        // (measured on 24 H2O in cc-pVDZ: (ps|ss) 27.8 -> 24.7 ms, while
+        // (ss|ss) 15.2 -> 16.0 and (ps|ps) 11.3 -> 12.0 ms)
+        if (b->ctx->twins && qbx_class_twin[cls] && X != class_pc(cls, 1) && !b->twins[X].empty()) {
+            std::vector<int> start64(nx, 0);
+            twin_tasks(b, cls, thr, mode, cut, b->w64[cls], start64, st);
+            if (b->w64[cls].ngroup > 0) {
+                TaskCounts ts;
+                task_counts(b, cls, thr, mode, cut, ts, &start64);
+                upload_tasks(b->s64[cls], b->s32[cls], ts, nx, st);
+                b->use_twin[cls] = true;
+            }
+        }
     }
     QBX_HIP(hipStreamSynchronize(st));
     b->tasks_valid = true;
@@ -908,28 +1035,37 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
         a.xcd_local = ctx->xcd_local ? 1 : 0;
         a.jk_stride = NN * nset;
         a.nset = nset;
-        const TaskSet& t64 = b->t64[cls];
-        a.prefix = t64.prefix.p;
-        a.koff = t64.koff.p;
-        a.cpre = t64.cpre.p;
-        a.spre = t64.spre.p;
-        a.nsuper = t64.nsuper;
+        // twin groups where their LDS rows ([set][member][A and B][N]) fit a block
+        bool twin = b->use_twin[cls];
+        if (twin) {
+            const size_t rows = (size_t)nset * b->w64[cls].twmax *
+                                (qbx_ncart(qbx_class_l[cls][0]) + qbx_ncart(qbx_class_l[cls][1])) * b->nbf *
+                                sizeof(double);
+            const size_t boys = (size_t)QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE * sizeof(double);
+            twin = rows <= (size_t)ROW_LDS_MAX && rows + boys <= (size_t)ctx->lds_max;
+        }
+        auto set_tasks = [&](const TaskSet& t, const TwinSet* w) {
+            a.prefix = t.prefix.p;
+            a.koff = t.koff.p;
+            a.cpre = t.cpre.p;
+            a.spre = t.spre.p;
+            a.nsuper = t.nsuper;
+            a.total = t.total;
+            a.nwave = t.nwave;
+            a.nbra = t.nbra;
+            a.gpre = w ? w->gpre.p : nullptr;
+            a.gmem = w ? w->gmem.p : nullptr;
+            a.ngroup = w ? w->ngroup : 0;
+            a.ntwtask = w ? w->ntask : 0;
+            a.twmax = w ? w->twmax : 0;
+        };
         a.lds_max = ctx->lds_max;
-        a.total = t64.total;
-        a.nwave = t64.nwave;
-        a.nbra = t64.nbra;
+        const TaskSet& t64 = twin ? b->s64[cls] : b->t64[cls];
+        set_tasks(t64, twin ? &b->w64[cls] : nullptr);
         attach_workspace(b, cls, std::max(t64.nwave, mode == QBX_PREC_MIXED ? b->t32[cls].nwave : 0), a);
         k_jk64[cls](a, ls);
         if (mode == QBX_PREC_MIXED) {
-            const TaskSet& t32 = b->t32[cls];
-            a.prefix = t32.prefix.p;
-            a.koff = t32.koff.p;
-            a.cpre = t32.cpre.p;
-            a.spre = t32.spre.p;
-            a.nsuper = t32.nsuper;
-            a.total = t32.total;
-            a.nwave = t32.nwave;
-            a.nbra = t32.nbra;
+            set_tasks(b->t32[cls], nullptr);
             k_jk32[cls](a, ls);
         }
         if (ctx->timing) QBX_HIP(hipEventRecord(ev[2 * cls + 1], st));
@@ -1022,6 +1158,10 @@ int qbx_init(int32_t device, qbx_ctx** out)
             const char* env = std::getenv("QBX_XCD_LOCAL");
             const bool off = env && env[0] == '0';
             c->xcd_local = ok && n >= 2 && !off;
+        }
+        {
+            const char* env = std::getenv("QBX_TWINS");
+            c->twins = !(env && env[0] == '0');
         }
         QBX_HIP(hipStreamSynchronize(c->stream));
         *out = c.release();

@@ -1,7 +1,8 @@
 // qbx_device.h -- device-side building blocks of libqbx_hip.so (gfx950).
 //
-// Boys function F_m(T): tabulated on T_i = i/10 (0 <= T < QBX_BOYS_TMAX) for
-// m = 0..QBX_BOYS_NCOL-1 together with exp(-T_i).  For T below the cut, the
+// Boys function F_m(T), one-electron kernels: tabulated on T_i = i/10
+// (0 <= T < QBX_BOYS_TMAX) for m = 0..QBX_BOYS_NCOL-1 together with exp(-T_i)
+// (global memory; the ERI kernels use the per-order LDS rows below).  For T below the cut, the
 // highest needed order comes from an 8-term Taylor expansion about the nearest
 // grid point (|T - T_i| <= 0.05, truncation < 1e-15 relative) and the lower
 // orders from the stable downward recurrence

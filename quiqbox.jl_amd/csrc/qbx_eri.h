@@ -440,6 +440,115 @@ __device__ __forceinline__ void flush_rows(double* krow, int nrowel, int nrow, i
     }
 }
 
+// J/K digest of one wave's contracted quartets for a wave-uniform bra:
+// kacc holds [e|cd) after the ket transfer.  J_ab is reduced across the wave,
+// J_cd goes out per lane through the sink, and the K updates (each with one
+// index on the bra) go to the LDS rows rows0 + set * rstride, laid out [A and
+// B functions][N].  f is the quartet's degeneracy weight (0 for dead lanes).
+template <int LA, int LB, int LC, int LD, typename R>
+__device__ __forceinline__ void digest_rows(const EriArgs& a, const R* kacc, const int* fa, const int* fb,
+                                            cptr<DevShell> SA, cptr<DevShell> SB, R ABx, R ABy, R ABz, int sC,
+                                            int sD, double f, bool live, double* rows0, int rstride,
+                                            const JKSink& sink, int lane)
+{
+    using Cls = EriClass<LA, LB, LC, LD, R>;
+    constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB), NC = qbx_ncart(LC), ND = qbx_ncart(LD);
+    constexpr int NCD = NC * ND;
+    const int Ni = a.N;
+    const size_t N = (size_t)a.N;
+    const DevShell* SC = a.sh + sC;
+    const DevShell* SD = a.sh + sD;
+    int fc[NC], fd[ND];
+    double scd[NCD];
+#pragma unroll
+    for (int n = 0; n < NC; ++n) fc[n] = SC->bf[n] < 0 ? 0 : SC->bf[n];
+#pragma unroll
+    for (int n = 0; n < ND; ++n) fd[n] = SD->bf[n] < 0 ? 0 : SD->bf[n];
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+        for (int id = 0; id < ND; ++id) scd[ic * ND + id] = SC->sc[ic] * SD->sc[id];
+    const double fj = 4.0 * f, fk = 2.0 * f;
+
+    for (int s = 0; s < a.nset; ++s) {
+        const double* __restrict__ DJ = a.D + (size_t)s * N * N;
+        const double* __restrict__ DK = a.D + (size_t)(a.nset + s) * N * N;
+        double* Js = sink.J + (size_t)s * N * N;
+        double* rows = rows0 + s * rstride;
+        double dcd[NCD], dbc[NB * NC], dbd[NB * ND];
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = DJ[fc[ic] * N + fd[id]];
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) dbc[ib * NC + ic] = DK[fb[ib] * N + fc[ic]];
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) dbd[ib * ND + id] = DK[fb[ib] * N + fd[id]];
+        double jcd[NCD], kbd[NB * ND], kbc[NB * NC];
+#pragma unroll
+        for (int n = 0; n < NCD; ++n) jcd[n] = 0.0;
+#pragma unroll
+        for (int n = 0; n < NB * ND; ++n) kbd[n] = 0.0;
+#pragma unroll
+        for (int n = 0; n < NB * NC; ++n) kbc[n] = 0.0;
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia) {
+            R out[NB * NCD];
+            Cls::hrr_bra(ia, kacc, out, ABx, ABy, ABz);
+            double dac[NC], dad[ND], kac[NC], kad[ND];
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) { dac[ic] = DK[fa[ia] * N + fc[ic]]; kac[ic] = 0.0; }
+#pragma unroll
+            for (int id = 0; id < ND; ++id) { dad[id] = DK[fa[ia] * N + fd[id]]; kad[id] = 0.0; }
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib) {
+                const double sab = SA->sc[ia] * SB->sc[ib];
+                const double dab = cst(DJ)[fa[ia] * N + fb[ib]];
+                double jab = 0.0;
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) {
+                        const int cd = ic * ND + id;
+                        const double v = (double)out[ib * NCD + cd] * (sab * scd[cd]);
+                        jab += v * dcd[cd];
+                        jcd[cd] += v * dab;
+                        kad[id] += v * dbc[ib * NC + ic];
+                        kac[ic] += v * dbd[ib * ND + id];
+                        kbd[ib * ND + id] += v * dac[ic];
+                        kbc[ib * NC + ic] += v * dad[id];
+                    }
+                jab = wave_sum(fj * jab);
+                if (lane == 0) sink.add(Js + fa[ia] * N + fb[ib], jab);
+            }
+            if (live) {
+#pragma unroll
+                for (int id = 0; id < ND; ++id) atomicAdd(rows + ia * Ni + fd[id], fk * kad[id]);
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic) atomicAdd(rows + ia * Ni + fc[ic], fk * kac[ic]);
+            }
+        }
+        if (live) {
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                for (int id = 0; id < ND; ++id) sink.add(Js + fc[ic] * N + fd[id], fj * jcd[ic * ND + id]);
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+                for (int id = 0; id < ND; ++id) atomicAdd(rows + (NA + ib) * Ni + fd[id], fk * kbd[ib * ND + id]);
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic) atomicAdd(rows + (NA + ib) * Ni + fc[ic], fk * kbc[ib * NC + ic]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // J/K with the bra's K rows in LDS.  A block of ROW_WAVES waves takes one bra
 // and up to ROW_CHUNKS of its 64-ket chunks.  Every K update of a quartet has
@@ -523,104 +632,180 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void eri_jk_rows_kernel(EriArgs a)
         R kacc[Cls::NE * NCD];
         Cls::hrr_ket(acc, kacc, (R)CDp[0], (R)CDp[1], (R)CDp[2]);
 
-        const DevShell* SC = a.sh + sC;
-        const DevShell* SD = a.sh + sD;
-        int fc[NC], fd[ND];
-        double scd[NCD];
-#pragma unroll
-        for (int n = 0; n < NC; ++n) fc[n] = SC->bf[n] < 0 ? 0 : SC->bf[n];
-#pragma unroll
-        for (int n = 0; n < ND; ++n) fd[n] = SD->bf[n] < 0 ? 0 : SD->bf[n];
-#pragma unroll
-        for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-            for (int id = 0; id < ND; ++id) scd[ic * ND + id] = SC->sc[ic] * SD->sc[id];
         double f = live ? 1.0 : 0.0;
         if (sA == sB) f *= 0.5;
         if (sC == sD) f *= 0.5;
         if (a.same && bi == ki) f *= 0.5;
-        const double fj = 4.0 * f, fk = 2.0 * f;
-
-        for (int s = 0; s < a.nset; ++s) {
-            const double* __restrict__ DJ = a.D + (size_t)s * N * N;
-            const double* __restrict__ DK = a.D + (size_t)(a.nset + s) * N * N;
-            double* Js = sink.J + (size_t)s * N * N;
-            double* rows = krow + s * NROW * Ni;
-            double dcd[NCD], dbc[NB * NC], dbd[NB * ND];
-#pragma unroll
-            for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-                for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = DJ[fc[ic] * N + fd[id]];
-#pragma unroll
-            for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-                for (int ic = 0; ic < NC; ++ic) dbc[ib * NC + ic] = DK[fb[ib] * N + fc[ic]];
-#pragma unroll
-            for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-                for (int id = 0; id < ND; ++id) dbd[ib * ND + id] = DK[fb[ib] * N + fd[id]];
-            double jcd[NCD], kbd[NB * ND], kbc[NB * NC];
-#pragma unroll
-            for (int n = 0; n < NCD; ++n) jcd[n] = 0.0;
-#pragma unroll
-            for (int n = 0; n < NB * ND; ++n) kbd[n] = 0.0;
-#pragma unroll
-            for (int n = 0; n < NB * NC; ++n) kbc[n] = 0.0;
-#pragma unroll
-            for (int ia = 0; ia < NA; ++ia) {
-                R out[NB * NCD];
-                Cls::hrr_bra(ia, kacc, out, ABx, ABy, ABz);
-                double dac[NC], dad[ND], kac[NC], kad[ND];
-#pragma unroll
-                for (int ic = 0; ic < NC; ++ic) { dac[ic] = DK[fa[ia] * N + fc[ic]]; kac[ic] = 0.0; }
-#pragma unroll
-                for (int id = 0; id < ND; ++id) { dad[id] = DK[fa[ia] * N + fd[id]]; kad[id] = 0.0; }
-#pragma unroll
-                for (int ib = 0; ib < NB; ++ib) {
-                    const double sab = SA->sc[ia] * SB->sc[ib];
-                    const double dab = cst(DJ)[fa[ia] * N + fb[ib]];
-                    double jab = 0.0;
-#pragma unroll
-                    for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-                        for (int id = 0; id < ND; ++id) {
-                            const int cd = ic * ND + id;
-                            const double v = (double)out[ib * NCD + cd] * (sab * scd[cd]);
-                            jab += v * dcd[cd];
-                            jcd[cd] += v * dab;
-                            kad[id] += v * dbc[ib * NC + ic];
-                            kac[ic] += v * dbd[ib * ND + id];
-                            kbd[ib * ND + id] += v * dac[ic];
-                            kbc[ib * NC + ic] += v * dad[id];
-                        }
-                    jab = wave_sum(fj * jab);
-                    if (lane == 0) sink.add(Js + fa[ia] * N + fb[ib], jab);
-                }
-                if (live) {
-#pragma unroll
-                    for (int id = 0; id < ND; ++id) atomicAdd(rows + ia * Ni + fd[id], fk * kad[id]);
-#pragma unroll
-                    for (int ic = 0; ic < NC; ++ic) atomicAdd(rows + ia * Ni + fc[ic], fk * kac[ic]);
-                }
-            }
-            if (live) {
-#pragma unroll
-                for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-                    for (int id = 0; id < ND; ++id) sink.add(Js + fc[ic] * N + fd[id], fj * jcd[ic * ND + id]);
-#pragma unroll
-                for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-                    for (int id = 0; id < ND; ++id) atomicAdd(rows + (NA + ib) * Ni + fd[id], fk * kbd[ib * ND + id]);
-#pragma unroll
-                for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-                    for (int ic = 0; ic < NC; ++ic) atomicAdd(rows + (NA + ib) * Ni + fc[ic], fk * kbc[ib * NC + ic]);
-            }
-        }
+        digest_rows<LA, LB, LC, LD, R>(a, kacc, fa, fb, SA, SB, ABx, ABy, ABz, sC, sD, f, live, krow, NROW * Ni,
+                                       sink, lane);
     }
     __syncthreads();
     flush_rows<NA, TPB>(krow, nrowel, NROW, Ni, a.sh[sA].bf, a.sh[sB].bf, sink, false);
+}
+
+// ---------------------------------------------------------------------------
+// J/K for a group of twin bras.  A block of TWIN_WAVES waves takes one group
+// and up to ROW_CHUNKS 64-ket chunks of the union of its members' ket ranges
+// (each member's range is a prefix of the same sorted ket list, or its FP32
+// tail).  The geometry and Boys values of every primitive quartet are computed
+// once, from the first member's primitive pairs; each member accumulates them
+// with its own c_a c_b factor (Kp) where the ket lies in its range.  Each
+// member then goes through the digest with its own shells and LDS K rows
+// [set][member][A and B functions][N].
+// ---------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ __launch_bounds__(64 * TWIN_WAVES) void eri_jk_twin_kernel(EriArgs a)
+{
+    using Cls = EriClass<LA, LB, LC, LD, R>;
+    constexpr int TPB = 64 * TWIN_WAVES;
+    constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB);
+    constexpr int LT = LA + LB + LC + LD;
+    constexpr int NCD = qbx_ncart(LC) * qbx_ncart(LD);
+    constexpr int NROW = NA + NB;
+    constexpr int TW = TWIN_MAX;
+    static_assert(ROW_CHUNKS % TWIN_WAVES == 0, "chunks per task must split evenly over the waves");
+    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
+    extern __shared__ double krow[];    // [nset][twmax][NROW][N]
+    const int Ni = a.N;
+    const int tw = a.twmax;
+    const int nrowel = a.nset * tw * NROW * Ni;
+    stage_boys<TPB>(sboys, a.boysl);
+    for (int i = threadIdx.x; i < nrowel; i += TPB) krow[i] = 0.0;
+    __syncthreads();
+    const double* sb = (const double*)sboys;
+
+    const int lane = (int)threadIdx.x & 63;
+    const int wib = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const cptr<int> gpre = cst(a.gpre);
+    const int task = (int)blockIdx.x;
+    int lo = 0, hi = a.ngroup;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (gpre[mid] <= task) lo = mid; else hi = mid;
+    }
+    const int g = lo;
+    const int sc = task - gpre[g];
+    const cptr<int> gm = cst(a.gmem + 3 * TW * g);
+    int bi[TW], klo[TW], khi[TW], sA[TW], sB[TW], ppo[TW];
+    int ulo = 0x7fffffff, uhi = 0;
+#pragma unroll
+    for (int t = 0; t < TW; ++t) {
+        bi[t] = t < tw ? gm[3 * t] : -1;
+        klo[t] = t < tw ? gm[3 * t + 1] : 0;
+        khi[t] = t < tw ? gm[3 * t + 2] : 0;
+        const int b = bi[t] < 0 ? bi[0] : bi[t];
+        sA[t] = cst(a.bra.shA)[b];
+        sB[t] = cst(a.bra.shB)[b];
+        ppo[t] = cst(a.bra.ppo)[b];
+        if (bi[t] >= 0 && khi[t] > klo[t]) {
+            ulo = min(ulo, klo[t]);
+            uhi = max(uhi, khi[t]);
+        }
+    }
+    const int bn = cst(a.bra.ppn)[bi[0]];
+    const cptr<PrimPair> BP = cst(a.pp + ppo[0]);
+    const cptr<double> ABp = cst(a.bra.AB + 3 * bi[0]);
+    const R ABx = (R)ABp[0], ABy = (R)ABp[1], ABz = (R)ABp[2];
+    const JKSink sink(a);
+
+    for (int r = 0; r < ROW_CHUNKS / TWIN_WAVES; ++r) {
+        const int chunk = (sc * (ROW_CHUNKS / TWIN_WAVES) + r) * TWIN_WAVES + wib;
+        if (ulo >= uhi || ulo + chunk * 64 >= uhi) break;
+        const int pos = ulo + chunk * 64 + lane;
+        const bool inrange = pos < uhi;
+        const int ki = inrange ? pos : ulo;
+        const int sC = a.ket.shA[ki], sD = a.ket.shB[ki];
+        bool live[TW];
+        bool any = false;
+#pragma unroll
+        for (int t = 0; t < TW; ++t) {
+            bool l = inrange && bi[t] >= 0 && pos >= klo[t] && pos < khi[t];
+            if (l && a.pdmax) {
+                const double* dm = a.pdmax;
+                const int ns = a.nshell;
+                double d = fmax(fmax(dm[sA[t] * ns + sB[t]], dm[sC * ns + sD]),
+                                fmax(dm[sA[t] * ns + sC], dm[sA[t] * ns + sD]));
+                d = fmax(d, fmax(dm[sB[t] * ns + sC], dm[sB[t] * ns + sD]));
+                if (cst(a.bra.Q)[bi[t]] * a.ket.Q[ki] * d < a.dthr) l = false;
+            }
+            live[t] = l;
+            any = any || l;
+        }
+        const int kn = any ? a.ket.ppn[ki] : 0;
+        if (__ballot(kn > 0) == 0) continue;
+
+        R acc[TW][Cls::NACC];
+#pragma unroll
+        for (int t = 0; t < TW; ++t)
+#pragma unroll
+            for (int n = 0; n < Cls::NACC; ++n) acc[t][n] = (R)0;
+        const PrimPair* KP = a.pp + a.ket.ppo[ki];
+        for (int v = 0; v < kn; ++v) {
+            const KetPrim Q = load_ket(KP + v);
+            for (int u = 0; u < bn; ++u) {
+                const double p = BP[u].p, inv2p = BP[u].inv2p;
+                const double PQx = BP[u].Px - Q.Qx, PQy = BP[u].Py - Q.Qy, PQz = BP[u].Pz - Q.Qz;
+                const double pq = p + Q.q;
+                const double s = rsqrt(pq);
+                const double inv = s * s;
+                const double rho = p * Q.q * inv;
+                const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+                const double pref0 = 34.986836655249725693 * Q.Kq * s;
+                double Fd[LT + 1];
+                boys_lds<LT>(T, Fd, sb);
+                const double rho_p = Q.q * inv, rho_q = p * inv;
+#pragma unroll
+                for (int t = 0; t < TW; ++t) {
+                    if (bi[t] < 0) continue;
+                    const double w = live[t] ? pref0 * cst(a.pp + ppo[t])[u].Kp : 0.0;
+                    R F[LT + 1];
+#pragma unroll
+                    for (int m = 0; m <= LT; ++m) F[m] = (R)(Fd[m] * w);
+                    Cls::vrr(F, acc[t], (R)BP[u].PAx, (R)BP[u].PAy, (R)BP[u].PAz,
+                             (R)(-rho_p * PQx), (R)(-rho_p * PQy), (R)(-rho_p * PQz),
+                             (R)Q.QCx, (R)Q.QCy, (R)Q.QCz,
+                             (R)(rho_q * PQx), (R)(rho_q * PQy), (R)(rho_q * PQz),
+                             (R)inv2p, (R)Q.inv2q, (R)(0.5 * inv), (R)rho_p, (R)rho_q);
+                }
+            }
+        }
+        const double* CDp = a.ket.AB + 3 * ki;
+#pragma unroll
+        for (int t = 0; t < TW; ++t) {
+            if (bi[t] < 0 || __ballot(live[t]) == 0) continue;
+            R kacc[Cls::NE * NCD];
+            Cls::hrr_ket(acc[t], kacc, (R)CDp[0], (R)CDp[1], (R)CDp[2]);
+            const cptr<DevShell> SA = cst(a.sh + sA[t]);
+            const cptr<DevShell> SB = cst(a.sh + sB[t]);
+            int fa[NA], fb[NB];
+#pragma unroll
+            for (int n = 0; n < NA; ++n) fa[n] = SA->bf[n] < 0 ? 0 : SA->bf[n];
+#pragma unroll
+            for (int n = 0; n < NB; ++n) fb[n] = SB->bf[n] < 0 ? 0 : SB->bf[n];
+            double f = live[t] ? 1.0 : 0.0;
+            if (sA[t] == sB[t]) f *= 0.5;
+            if (sC == sD) f *= 0.5;
+            if (a.same && bi[t] == ki) f *= 0.5;
+            digest_rows<LA, LB, LC, LD, R>(a, kacc, fa, fb, SA, SB, ABx, ABy, ABz, sC, sD, f, live[t],
+                                           krow + t * NROW * Ni, tw * NROW * Ni, sink, lane);
+        }
+    }
+    __syncthreads();
+    // each nonzero row element to K once
+    const size_t N = (size_t)Ni;
+    for (int i = threadIdx.x; i < nrowel; i += TPB) {
+        const double v = krow[i];
+        if (v == 0.0) continue;
+        const int s = i / (tw * NROW * Ni);
+        int rem = i - s * tw * NROW * Ni;
+        const int t = rem / (NROW * Ni);
+        rem -= t * NROW * Ni;
+        const int row = rem / Ni, col = rem - row * Ni;
+        const int b = gm[3 * t];
+        const int sh = row < NA ? cst(a.bra.shA)[b] : cst(a.bra.shB)[b];
+        const int fr = max(a.sh[sh].bf[row < NA ? row : row - NA], 0);
+        sink.add(sink.K + (size_t)s * N * N + (size_t)fr * N + col, v);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1030,6 +1215,29 @@ void launch_eri(const EriArgs& a, hipStream_t s)
         QBX_HIP(hipGetLastError());
     } else {
         if constexpr (MODE == MODE_JK) {
+            if (a.ngroup > 0) {
+                if constexpr (EriClass<LA, LB, LC, LD, R>::NACC <= TWIN_MAX_ACC) {
+                    const size_t tshm =
+                        (size_t)a.nset * a.twmax * (qbx_ncart(LA) + qbx_ncart(LB)) * a.N * sizeof(double);
+                    static const bool twin_ok = [] {
+                        const bool ok = hipFuncSetAttribute((const void*)eri_jk_twin_kernel<LA, LB, LC, LD, R>,
+                                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                            ROW_LDS_MAX) == hipSuccess;
+                        (void)hipGetLastError();
+                        return ok;
+                    }();
+                    if (!twin_ok || tshm > (size_t)ROW_LDS_MAX ||
+                        tshm + QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE * sizeof(double) > (size_t)a.lds_max)
+                        throw QbxError(QBX_EINTERNAL, "twin-bra groups given where their LDS rows do not fit");
+                    if (a.ntwtask > 0) {
+                        hipLaunchKernelGGL((eri_jk_twin_kernel<LA, LB, LC, LD, R>), dim3((unsigned)a.ntwtask),
+                                           dim3(64 * TWIN_WAVES), tshm, s, a);
+                        QBX_HIP(hipGetLastError());
+                    }
+                } else {
+                    throw QbxError(QBX_EINTERNAL, "twin-bra groups given to a class without a twin kernel");
+                }
+            }
             const size_t shm = (size_t)a.nset * (qbx_ncart(LA) + qbx_ncart(LB)) * a.N * sizeof(double);
             const size_t boys_lds = QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE * sizeof(double);
             // the attribute call fails where the device cannot give a block

@@ -16,6 +16,14 @@ enum EriMode { MODE_JK = 0, MODE_DENSE = 1, MODE_SCHWARZ = 2 };
 constexpr int ROW_WAVES = QBX_ROW_WAVES;
 constexpr int ROW_CHUNKS = 16;
 constexpr int ROW_LDS_MAX = 64 * 1024;
+// Twin bras: shell pairs of one pair class with the same primitive geometry
+// and primitive pairs, differing only in contraction coefficients (shells
+// that share centre, L and exponents, as the 1s and 2s contractions of
+// cc-pVDZ oxygen).  A group of up to TWIN_MAX of them is evaluated together
+// by classes with at most TWIN_MAX_ACC accumulators.
+constexpr int TWIN_MAX = 4;
+constexpr int TWIN_WAVES = 8;   // waves per block of the twin kernel (its K rows take up to 61 KB of LDS)
+constexpr int TWIN_MAX_ACC = 9;
 
 struct PairClassDev {
     int n;
@@ -39,6 +47,11 @@ struct EriArgs {
     const int* cpre;          // [nbra+1] prefix over bras of ceil(kets / 64): waves
     const int* spre;          // [nbra+1] prefix over bras of ceil(waves / ROW_CHUNKS): K-row blocks
     int nsuper;               // number of K-row blocks
+    const int* gpre;          // [ngroup+1] twin groups: prefix of tasks (ROW_CHUNKS chunks of the union range)
+    const int* gmem;          // [ngroup][TWIN_MAX][3]: bra, first ket, end of kets (bra < 0: no member)
+    int ngroup;               // twin groups of this class (0: none)
+    int ntwtask;              // their tasks
+    int twmax;                // largest group (LDS rows per set)
     int lds_max;              // LDS bytes a block may use on this device
     int64_t total;            // Schwarz mode: number of diagonal pairs
     int nwave;                // J/K and dense modes: number of waves
