@@ -117,6 +117,18 @@ __device__ __forceinline__ void stage_boys(double2* __restrict__ dst, const doub
     for (int i = threadIdx.x; i < QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2; i += TPB) dst[i] = s[i];
 }
 
+// 1/sqrt(x) for finite x > 0 (p + q of a primitive quartet): the hardware
+// estimate r refined by the second-order term of the binomial series,
+// 1/sqrt(x) = r (1 - e)^(-1/2) = r (1 + e/2 + 3e^2/8 + ...) with e = 1 - x r^2.
+// The library rsqrt() applies the same correction and then re-checks the
+// estimate for zero, infinite and NaN inputs, which cannot occur here.
+__device__ __forceinline__ double rsqrt_pos(double x)
+{
+    const double r = __builtin_amdgcn_rsq(x);
+    const double e = fma(r * -x, r, 1.0);
+    return fma(r * e, fma(e, 0.375, 0.5), r);
+}
+
 // The ket-pair quantities one primitive quartet reads.
 struct KetPrim {
     double q, inv2q, Qx, Qy, Qz, QCx, QCy, QCz, Kq;
@@ -160,7 +172,7 @@ __device__ __forceinline__ void prim_loop(R* __restrict__ acc, BraPtr BP, int bn
             const double p = BP[u].p, inv2p = BP[u].inv2p, Px = BP[u].Px, Py = BP[u].Py, Pz = BP[u].Pz;
             const double PAx = BP[u].PAx, PAy = BP[u].PAy, PAz = BP[u].PAz, Kp = BP[u].Kp;
             const double pq = p + Q.q;
-            const double s = rsqrt(pq);
+            const double s = rsqrt_pos(pq);
             const double inv = s * s;
             const double PQx = Px - Q.Qx, PQy = Py - Q.Qy, PQz = Pz - Q.Qz;
             const double rho = p * Q.q * inv;
@@ -746,7 +758,7 @@ __global__ __launch_bounds__(64 * TWIN_WAVES) void eri_jk_twin_kernel(EriArgs a)
                 const double p = BP[u].p, inv2p = BP[u].inv2p;
                 const double PQx = BP[u].Px - Q.Qx, PQy = BP[u].Py - Q.Qy, PQz = BP[u].Pz - Q.Qz;
                 const double pq = p + Q.q;
-                const double s = rsqrt(pq);
+                const double s = rsqrt_pos(pq);
                 const double inv = s * s;
                 const double rho = p * Q.q * inv;
                 const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
