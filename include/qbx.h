@@ -73,6 +73,13 @@ int  qbx_kinetic(qbx_ctx* ctx, const qbx_basis* basis, double* T);
    test charge -1 as CoulombMultiPointPotential (Samplers.jl:56-61) */
 int  qbx_nuclear(qbx_ctx* ctx, const qbx_basis* basis, int32_t n_nuc, const double* Z,
                  const double* R, double* V);
+/* Own-nucleus attraction (no reference counterpart; the learned guess's pair
+   features): V[i][j] = <i| -Z_n / |r - R_n| |j> for the nucleus n at the centre
+   of function i alone, or 0 where no nucleus sits there (centres match within
+   1e-10 bohr).  V is not symmetric: V^T holds the column function's nucleus.
+   One call replaces one qbx_nuclear call per nucleus. */
+int  qbx_nuclear_own(qbx_ctx* ctx, const qbx_basis* basis, int32_t n_nuc, const double* Z,
+                     const double* R, double* V);
 /* coreHamiltonian(nuc, coords, bs)  Interface.jl:288-312 = kinetic + nuclear */
 int  qbx_core_hamiltonian(qbx_ctx* ctx, const qbx_basis* basis, int32_t n_nuc,
                           const double* Z, const double* R, double* H);

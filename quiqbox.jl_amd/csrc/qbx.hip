@@ -241,7 +241,7 @@ void build_boys_table(std::vector<double>& tab, std::vector<double>& et, std::ve
 // ---------------------------------------------------------------------------
 // kernels: one-electron integrals
 // ---------------------------------------------------------------------------
-enum OneE { ONE_OVERLAP = 0, ONE_KINETIC = 1, ONE_NUCLEAR = 2, ONE_MULTIPOLE = 3 };
+enum OneE { ONE_OVERLAP = 0, ONE_KINETIC = 1, ONE_NUCLEAR = 2, ONE_MULTIPOLE = 3, ONE_NUCLEAR_OWN = 4 };
 
 struct OneArgs {
     const DevShell* sh;
@@ -257,6 +257,7 @@ struct OneArgs {
     int nnuc;
     const double* Z;
     const double* R;
+    const int* own;        // ONE_NUCLEAR_OWN: nucleus at each shell's centre (-1: none)
     double M[3];
     int deg[3];
     double* out;
@@ -447,12 +448,26 @@ __global__ __launch_bounds__(256) void one_e_kernel(OneArgs a)
     qbx_cart(A.L, ca, la);
     qbx_cart(B.L, cb, lb);
     const double AB[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
-    double acc = 0.0;
+    double acc = 0.0, accB = 0.0;
     for (int u = 0; u < a.pn[pair]; ++u) {
         const PrimPair P = a.pp[a.po[pair] + u];
         const double PA[3] = {P.PAx, P.PAy, P.PAz};
         const double PB[3] = {P.PAx + AB[0], P.PAy + AB[1], P.PAz + AB[2]};
-        if (KIND == ONE_NUCLEAR) {
+        if (KIND == ONE_NUCLEAR_OWN) {
+            // row function on A: A's own nucleus; row function on B: B's
+            const int Lt = A.L + B.L;
+            for (int side = 0; side < 2; ++side) {
+                const int n = a.own[side ? a.pB[pair] : a.pA[pair]];
+                if (n < 0) continue;
+                const double PC[3] = {P.Px - a.R[3 * n], P.Py - a.R[3 * n + 1], P.Pz - a.R[3 * n + 2]};
+                const double T = P.p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+                double F[2 * QBX_MAXL + 1];
+                qbx_boys_rt(Lt, T, F, a.boys, a.bexp);
+                const double v = -2.0 * kPi * P.Kp * a.Z[n] * coulomb1_comp(la, lb, PA, PC, AB, P.inv2p, F, Lt);
+                if (side) accB += v;
+                else acc += v;
+            }
+        } else if (KIND == ONE_NUCLEAR) {
             const int Lt = A.L + B.L;
             const double Pc[3] = {P.Px, P.Py, P.Pz};
             double v = 0.0;
@@ -506,7 +521,7 @@ __global__ __launch_bounds__(256) void one_e_kernel(OneArgs a)
     }
     acc *= A.sc[ca] * B.sc[cb];
     a.out[(size_t)ia * a.N + ib] = acc;
-    a.out[(size_t)ib * a.N + ia] = acc;
+    a.out[(size_t)ib * a.N + ia] = KIND == ONE_NUCLEAR_OWN ? accB * (A.sc[ca] * B.sc[cb]) : acc;
 }
 
 // max over the density blocks of shell pair (s1, s2): |D[s][i][j]|, all 2*nset
@@ -946,7 +961,8 @@ void run_one_e(qbx_ctx* ctx, const qbx_basis* b, int kind, int nnuc, const doubl
     a.boys = ctx->boys.p;
     a.bexp = ctx->bexp.p;
     a.out = dout.p;
-    if (kind == ONE_NUCLEAR) {
+    DevBuf<int> down;
+    if (kind == ONE_NUCLEAR || kind == ONE_NUCLEAR_OWN) {
         if (nnuc <= 0 || !Z || !R) throw QbxError(QBX_EINVAL, "nuclear attraction needs at least one nucleus");
         std::vector<double> z(Z, Z + nnuc), r(R, R + 3 * nnuc);
         dZ.upload(z, st);
@@ -954,6 +970,18 @@ void run_one_e(qbx_ctx* ctx, const qbx_basis* b, int kind, int nnuc, const doubl
         a.nnuc = nnuc;
         a.Z = dZ.p;
         a.R = dR.p;
+    }
+    if (kind == ONE_NUCLEAR_OWN) {
+        std::vector<int> own(b->shells.size(), -1);
+        for (size_t s = 0; s < b->shells.size(); ++s)
+            for (int n = 0; n < nnuc; ++n)
+                if (std::fabs(b->shells[s].c[0] - R[3 * n]) + std::fabs(b->shells[s].c[1] - R[3 * n + 1]) +
+                        std::fabs(b->shells[s].c[2] - R[3 * n + 2]) <= 1e-10) {
+                    own[s] = n;
+                    break;
+                }
+        down.upload(own, st);
+        a.own = down.p;
     }
     if (kind == ONE_MULTIPOLE) {
         for (int t = 0; t < 3; ++t) {
@@ -969,6 +997,9 @@ void run_one_e(qbx_ctx* ctx, const qbx_basis* b, int kind, int nnuc, const doubl
         case ONE_OVERLAP: hipLaunchKernelGGL(one_e_kernel<ONE_OVERLAP>, dim3(blocks), dim3(256), 0, st, a); break;
         case ONE_KINETIC: hipLaunchKernelGGL(one_e_kernel<ONE_KINETIC>, dim3(blocks), dim3(256), 0, st, a); break;
         case ONE_NUCLEAR: hipLaunchKernelGGL(one_e_kernel<ONE_NUCLEAR>, dim3(blocks), dim3(256), 0, st, a); break;
+        case ONE_NUCLEAR_OWN:
+            hipLaunchKernelGGL(one_e_kernel<ONE_NUCLEAR_OWN>, dim3(blocks), dim3(256), 0, st, a);
+            break;
         default: hipLaunchKernelGGL(one_e_kernel<ONE_MULTIPOLE>, dim3(blocks), dim3(256), 0, st, a); break;
         }
         QBX_HIP(hipGetLastError());
@@ -1410,6 +1441,11 @@ int qbx_kinetic(qbx_ctx* ctx, const qbx_basis* b, double* T)
 int qbx_nuclear(qbx_ctx* ctx, const qbx_basis* b, int32_t n_nuc, const double* Z, const double* R, double* V)
 {
     return guarded([&] { run_one_e(ctx, b, ONE_NUCLEAR, n_nuc, Z, R, nullptr, nullptr, V); });
+}
+
+int qbx_nuclear_own(qbx_ctx* ctx, const qbx_basis* b, int32_t n_nuc, const double* Z, const double* R, double* V)
+{
+    return guarded([&] { run_one_e(ctx, b, ONE_NUCLEAR_OWN, n_nuc, Z, R, nullptr, nullptr, V); });
 }
 
 int qbx_core_hamiltonian(qbx_ctx* ctx, const qbx_basis* b, int32_t n_nuc, const double* Z, const double* R,

@@ -32,6 +32,7 @@ SIGNATURES = {
     "qbx_overlap": (_i32, [_vp, _vp, _pd]),
     "qbx_kinetic": (_i32, [_vp, _vp, _pd]),
     "qbx_nuclear": (_i32, [_vp, _vp, _i32, _pd, _pd, _pd]),
+    "qbx_nuclear_own": (_i32, [_vp, _vp, _i32, _pd, _pd, _pd]),
     "qbx_core_hamiltonian": (_i32, [_vp, _vp, _i32, _pd, _pd, _pd]),
     "qbx_multipole": (_i32, [_vp, _vp, _pd, _pi, _pd]),
     "qbx_eri_dense": (_i32, [_vp, _vp, _pd]),

@@ -133,6 +133,16 @@ class GPUBasis:
     def overlap(self):
         return self._mat_out(_lib.load().qbx_overlap)
 
+    def orthonormalization(self):
+        """X = S^{-1/2} of this basis (HartreeFock.jl:39-42), computed once per
+        basis (read-only array; every SCF and guess on the basis shares it)."""
+        if getattr(self, "_X", None) is None:
+            from .hartree_fock import getOrthonormalization
+            X = getOrthonormalization(self.overlap())
+            X.flags.writeable = False
+            self._X = X
+        return self._X
+
     def kinetic(self):
         return self._mat_out(_lib.load().qbx_kinetic)
 
@@ -140,6 +150,16 @@ class GPUBasis:
         Z = _lib.f64(Z)
         R = _lib.f64(R).reshape(-1)
         return self._mat_out(_lib.load().qbx_nuclear, len(Z), _lib.dp(Z), _lib.dp(R))
+
+    def nuclear_own(self, Z, R):
+        """V[i][j] = attraction of the nucleus at function i's centre alone
+        (qbx_nuclear_own; zero rows where no nucleus sits).  Standard bases only:
+        the per-function attribution does not survive a basis transform."""
+        if self.T is not None:
+            raise ValueError("own-nucleus attraction needs a standard (single-centre) basis")
+        Z = _lib.f64(Z)
+        R = _lib.f64(R).reshape(-1)
+        return self._mat_out(_lib.load().qbx_nuclear_own, len(Z), _lib.dp(Z), _lib.dp(R))
 
     def core_hamiltonian(self, Z, R):
         Z = _lib.f64(Z)

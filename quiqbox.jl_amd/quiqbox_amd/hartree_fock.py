@@ -985,7 +985,7 @@ def initializeHartreeFock(initCfg, basis, system, twoBody=None, schwarz=1e-12, p
     if twoBody is None:
         twoBody = gb.two_body(schwarz, precision, incremental=incremental) if incremental else \
             gb.two_body(schwarz, precision)
-    X = getOrthonormalization(S)
+    X = gb.orthonormalization() if hasattr(gb, "orthonormalization") else getOrthonormalization(S)
     config = ElecHamiltonianConfig(spin, X, S, Hcore, twoBody)
     HFT = initCfg.HFT
     if initCfg.method == "Direct":

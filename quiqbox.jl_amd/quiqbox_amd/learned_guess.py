@@ -34,7 +34,7 @@ import numpy as np
 
 from . import _lib
 from .engine import GPUBasis
-from .hartree_fock import (HFconfig, OrbCoeffInitialConfig, RCHartreeFock, SCFconfig, getC, getOrthonormalization,
+from .hartree_fock import (HFconfig, OrbCoeffInitialConfig, RCHartreeFock, SCFconfig, getC,
                            runHartreeFock)
 
 NMAX = 15                     # functions per atom in the padded blocks (cc-pVDZ oxygen: 15 Cartesian)
@@ -88,20 +88,20 @@ def _ext(M):
 
 def pair_features(nuc, blocks, S, H):
     """Raw features (n_atoms^2 x N_FEAT) of all ordered atom pairs.  H is
-    (T, Vn): the kinetic matrix and the attraction matrix of every nucleus."""
-    T, Vn = H
+    (T, O): the kinetic matrix and the own-nucleus attraction O[i][j] (the
+    nucleus at function i's centre alone, GPUBasis.nuclear_own), so that the
+    pair (A, B) gets T + V(A) + V(B) = T + O + O^T on its block, and the pair
+    (A, A) gets T + V(A)."""
+    T, O = H
     na = len(blocks)
     n = S.shape[0]
     idx = _padded_index(blocks, n)
     I, J = idx[:, None, :, None], idx[None, :, None, :]
     Sb = _ext(S)[I, J]                                  # (na, na, NMAX, NMAX)
-    Hb = _ext(T)[I, J]
-    for a in range(na):
-        Va = _ext(Vn[a])
-        Hb[a] += Va[idx[a][None, :, None], idx[:, None, :]]       # V(A)_AB for every B
-        Vab = Va[idx[:, :, None], idx[a][None, None, :]]          # V(A)_BA for every B
-        Vab[a] = 0.0                                               # the pair (A, A) holds V(A) once
-        Hb[:, a] += Vab
+    OTb = _ext(O.T)[I, J]
+    Hb = _ext(T)[I, J] + _ext(O)[I, J] + OTb
+    diag = np.arange(na)
+    Hb[diag, diag] -= OTb[diag, diag]                   # V(A) once on the atom's own block
     X = np.zeros((na * na, N_FEAT))
     X[:, :NMAX * NMAX] = Sb.reshape(na * na, -1)
     X[:, NMAX * NMAX:2 * NMAX * NMAX] = Hb.reshape(na * na, -1)
@@ -134,14 +134,13 @@ def assemble_density(blocks, Y, n):
 
 
 def system_data(nuc, gb):
-    """S, (T, per-nucleus attraction matrices), atom blocks; one-electron
-    integrals from the GPU kernels."""
+    """S, (T, own-nucleus attraction), atom blocks; one-electron integrals from
+    the GPU kernels."""
     S = gb.overlap()
     T = gb.kinetic()
-    Z, R = nuc.charges, nuc.coord_array
-    Vn = [gb.nuclear(Z[k:k + 1], R[k:k + 1]) for k in range(len(Z))]
     blocks = atom_blocks(nuc, gb)
-    return S, (T, Vn), blocks
+    O = gb.nuclear_own(nuc.charges, nuc.coord_array)
+    return S, (T, O), blocks
 
 
 class _Dev:
@@ -319,10 +318,9 @@ class LearnedGuess:
         if S is None:
             S, H, _ = system_data(nuc, gb)
         Dt = self.predict_density(nuc, gb, S, H)
-        T, Vn = H
-        Hcore = T + sum(Vn)
+        Hcore = gb.core_hamiltonian(nuc.charges, nuc.coord_array)
         G = gb.jk(Dt, 0.5 * Dt, schwarz)
-        X = getOrthonormalization(S)
+        X = gb.orthonormalization()
         return getC(X, Hcore + G), Dt
 
 

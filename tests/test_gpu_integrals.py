@@ -115,3 +115,28 @@ def test_nonstandard_composite_basis():
     bs = [mixed] + qb.genGaussTypeOrbSeq((0.3, 0.0, 0.0), "H", "STO-3G") + \
         qb.genGaussTypeOrbSeq((0.0, 1.2, 0.0), "Li", "3-21G")
     _compare_all(bs, [1.0, 3.0], [(0.3, 0, 0), (0, 1.2, 0)], label="non-standard")
+
+
+def test_own_nucleus_attraction_matches_per_nucleus_matrices():
+    """qbx_nuclear_own (the learned guess's features): row i is the attraction
+    of the nucleus at function i's centre alone; a function on no nucleus
+    (here a ghost s shell) has a zero row."""
+    from quiqbox_amd.systems import water_box
+    nuc, bs = water_box(2, 5, basis="cc-pVDZ")
+    bs = bs + qb.genGaussTypeOrbSeq((9.0, -3.0, 1.5), "H", "STO-3G")
+    gb = qb.GPUBasis(bs)
+    Z, R = nuc.charges, nuc.coord_array
+    O = gb.nuclear_own(Z, R)
+    flat = gb.flat
+    first = np.asarray(flat.bf_prim)[np.asarray(flat.bf_ptr)[:-1]]
+    cen = np.asarray(flat.prim_cen).reshape(-1, 3)[first]
+    Vn = [gb.nuclear(Z[k:k + 1], R[k:k + 1]) for k in range(len(Z))]
+    ref = np.zeros_like(O)
+    for i in range(gb.n):
+        d = np.linalg.norm(R - cen[i], axis=1)
+        if d.min() < 1e-10:
+            ref[i] = Vn[int(np.argmin(d))][i]
+    err = float(np.abs(O - ref).max())
+    print("[gpu-test] own-nucleus attraction: max |O - V(owner)| = %.3e" % err, flush=True)
+    assert err < 1e-13 * max(1.0, float(np.abs(ref).max()))
+    assert not O[-1].any()

@@ -1,6 +1,7 @@
 """The learned guess's pair features, targets and density assembly (vectorized
 over atom pairs) against their definition written pair by pair: the S block,
-the local core Hamiltonian T + V(A) (+ V(B) for B != A), element flags,
+the local core Hamiltonian T + V(A) (+ V(B) for B != A, each nucleus's own
+attraction matrix V symmetric) built from the own-nucleus matrix O, element flags,
 A == B, R, exp(-R), 1/(1+R), zero-padded to NMAX functions per atom."""
 import numpy as np
 
@@ -20,12 +21,18 @@ def _case(seed=0):
         k += z
     n = sum(sizes)
     mats = [rng.normal(size=(n, n)) for _ in range(2 + len(syms))]
-    return nuc, blocks, n, mats[0], mats[1], mats[2:]
+    S, T, Vn = mats[0], mats[1], [0.5 * (m + m.T) for m in mats[2:]]
+    # own-nucleus attraction: row i takes the attraction of function i's atom
+    owner = np.empty(n, dtype=int)
+    for a, b in enumerate(blocks):
+        owner[b] = a
+    O = np.stack([Vn[owner[i]][i] for i in range(n)])
+    return nuc, blocks, n, S, T, Vn, O
 
 
 def test_pair_features_follow_the_pairwise_definition():
-    nuc, blocks, n, S, T, Vn = _case()
-    X = lg.pair_features(nuc, blocks, S, (T, Vn))
+    nuc, blocks, n, S, T, Vn, O = _case()
+    X = lg.pair_features(nuc, blocks, S, (T, O))
     na = len(blocks)
     R = nuc.coord_array
     o = 2 * lg.NMAX * lg.NMAX
@@ -49,7 +56,7 @@ def test_pair_features_follow_the_pairwise_definition():
 
 
 def test_targets_and_assembly_are_inverse_block_maps():
-    nuc, blocks, n, S, T, Vn = _case(1)
+    nuc, blocks, n, S, T, Vn, O = _case(1)
     rng = np.random.default_rng(2)
     D = rng.normal(size=(n, n))
     D = D + D.T
