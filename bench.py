@@ -264,6 +264,31 @@ def gradient_section(steps):
             "energy_first": hist[0], "energy_last": hist[-1]}
 
 
+def mo_section(n, reps=3):
+    """changeOrbitalBasis (Interface.jl:376-380) of a dense n^4 two-body tensor
+    with an n x n coefficient matrix: four quarter transformations, each one
+    FP64 matrix-core GEMM of 2 n^5 flops.  Random values (the transform needs no
+    index symmetry); host arrays in and out, as the reference's interface hands
+    them over, so the wall time includes the PCIe copies."""
+    import quiqbox_amd as qb
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((n,) * 4)
+    C = rng.standard_normal((n, n))
+    ctx = qb.default_context()
+    ctx.mo_transform(A, C)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ctx.mo_transform(A, C)
+        ts.append(time.perf_counter() - t0)
+    t = min(ts)
+    flops = 4 * 2.0 * n ** 5
+    log("mo transform n %d: %.1f ms wall (%.2f GB each way over PCIe), %.2f TFLOP/s including copies"
+        % (n, 1e3 * t, 8.0 * n ** 4 / 1e9, flops / t / 1e12))
+    return {"workload": "two-body tensor %d^4 FP64, square coefficient matrix" % n, "wall_ms": 1e3 * t,
+            "gemm_flops": flops, "tflops_incl_pcie": flops / t / 1e12, "host_bytes_each_way": 8 * n ** 4}
+
+
 def rhf_section(hip, cpu_seconds, no_cpu):
     """RHF wall time to |dE| < 1e-8 Eh on the 600-function water cluster: plain
     FP64 and mixed precision + incremental builds, plus that system's FP64 Fock
@@ -344,6 +369,8 @@ def main():
     ap.add_argument("--learned-epochs", type=int, default=300)
     ap.add_argument("--grad-steps", type=int, default=100,
                     help="basis-optimization steps of the gradient section (0: skip it)")
+    ap.add_argument("--mo-size", type=int, default=128,
+                    help="orbital-basis change of an n^4 two-body tensor (0: skip it)")
     ap.add_argument("--serial", action="store_true",
                     help="run the class kernels of each build one after another on one stream (for profiling)")
     args = ap.parse_args()
@@ -451,6 +478,8 @@ def main():
             out["learned_guess"] = learned_section(*rhf_section.last, args.learned_systems, args.learned_epochs)
         if args.grad_steps > 0:
             out["gradient"] = gradient_section(args.grad_steps)
+    if args.mo_size > 0:
+        out["mo_transform"] = mo_section(args.mo_size)
     print(json.dumps(out), flush=True)
 
 

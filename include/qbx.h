@@ -91,6 +91,25 @@ int  qbx_multipole(qbx_ctx* ctx, const qbx_basis* basis, const double* center,
    n_bf <= 256 (the tensor is n_bf^4 doubles). */
 int  qbx_eri_dense(qbx_ctx* ctx, const qbx_basis* basis, double* eri);
 
+/* changeOrbitalBasis(DbodyInt, C)  Interface.jl:376-380: one- or two-body
+   integrals in the orbital basis of the n_ao x n_mo coefficient matrix C,
+     rank 2: out[i][j]       = sum_ab   ints[a][b] C[a][i] C[b][j]
+     rank 4: out[i][j][k][l] = sum_abcd ints[a][b][c][d] C[a][i] C[b][j] C[c][k] C[d][l]
+   as `rank` successive quarter transformations, each one FP64 matrix-core
+   GEMM over the device copy (two device buffers of the largest intermediate,
+   n_ao^3 max(n_ao, n_mo) doubles for rank 4).  Row-major host arrays; the
+   tensor needs no index symmetry. */
+int  qbx_mo_transform(qbx_ctx* ctx, int32_t rank, int32_t n_ao, int32_t n_mo, const double* ints,
+                      const double* C, double* out);
+/* getJab  Interface.jl:382-393 (third result of changeOrbitalBasis with two
+   coefficient matrices, Interface.jl:405-407): the Coulomb matrix between
+   orbitals of different spin,
+     Jab[m][n] = sum_abcd eri[a][b][c][d] C1[a][m] C1[b][m] C2[c][n] C2[d][n],
+   C1 n_ao x n1, C2 n_ao x n2, as two GEMMs over the orbital-pair columns
+   C1[a][m] C1[b][m] and C2[c][n] C2[d][n]. */
+int  qbx_mo_coulomb_ab(qbx_ctx* ctx, int32_t n_ao, int32_t n1, int32_t n2, const double* eri,
+                       const double* C1, const double* C2, double* Jab);
+
 /* Fock two-electron part: getGcore (HartreeFock.jl:305-319) for n_set density
    pairs at once.  For each set s:
      G[s] = J[DJ[s]] - K[DK[s]],

@@ -205,3 +205,24 @@ def gcore(eri, DJ, DK):
     G = np.zeros((n, n))
     lib().qor_gcore(n, _dp(e), _dp(dj), _dp(dk), _dp(G))
     return G
+
+
+def change_orbital_basis(ints, C):
+    """changeOrbitalBasis (Interface.jl:376-380): every index of the one- or
+    two-body integrals contracted with the coefficient matrix C, index order
+    kept (out[i, j(, k, l)])."""
+    A = np.asarray(ints, dtype=np.float64)
+    c = np.asarray(C, dtype=np.float64)
+    for _ in range(A.ndim):
+        A = np.tensordot(A, c, axes=([0], [0]))   # contracted index leaves the front, new one goes last
+    return A
+
+
+def coulomb_ab(eri, C1, C2):
+    """getJab (Interface.jl:382-393): Jab[m, n] = sum_abcd eri[a, b, c, d]
+    C1[a, m] C1[b, m] C2[c, n] C2[d, n]."""
+    e = np.asarray(eri, dtype=np.float64)
+    n = e.shape[0]
+    k1 = np.einsum("am,bm->abm", C1, C1).reshape(n * n, -1)
+    k2 = np.einsum("cn,dn->cdn", C2, C2).reshape(n * n, -1)
+    return k1.T @ e.reshape(n * n, n * n) @ k2

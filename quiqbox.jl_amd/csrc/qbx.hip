@@ -1631,6 +1631,78 @@ int qbx_density_purify(qbx_ctx* ctx, int32_t n, int32_t nocc, const double* X, c
     });
 }
 
+int qbx_mo_transform(qbx_ctx* ctx, int32_t rank, int32_t n_ao, int32_t n_mo, const double* ints, const double* C,
+                     double* out)
+{
+    return guarded([&] {
+        if (!ctx) throw QbxError(QBX_EINVAL, "null context");
+        if (rank != 2 && rank != 4) throw QbxError(QBX_EINVAL, "rank must be 2 (one-body) or 4 (two-body)");
+        if (n_ao <= 0 || n_mo <= 0) throw QbxError(QBX_EINVAL, "n_ao and n_mo must be positive");
+        if (!ints || !C || !out) throw QbxError(QBX_EINVAL, "null array pointer");
+        // quarter q maps [p][rest] (p of n_ao, rest of rows[q]) to [rest][i]
+        size_t rows[4], big = 0;
+        for (int q = 0; q < rank; ++q) {
+            size_t r = 1;
+            for (int t = 0; t < rank - 1; ++t) r *= (t < rank - 1 - q) ? (size_t)n_ao : (size_t)n_mo;
+            if (r > (size_t)INT32_MAX) throw QbxError(QBX_EUNSUP, "tensor too large for the transform");
+            rows[q] = r;
+            big = std::max(big, r * std::max((size_t)n_ao, (size_t)n_mo));
+        }
+        size_t n_in = 1, n_out = 1;
+        for (int t = 0; t < rank; ++t) {
+            n_in *= (size_t)n_ao;
+            n_out *= (size_t)n_mo;
+        }
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        QBX_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        DevBuf<double> buf[2], dC;
+        buf[0].ensure(big);
+        buf[1].ensure(big);
+        dC.upload(std::vector<double>(C, C + (size_t)n_ao * n_mo), st);
+        QBX_HIP(hipMemcpyAsync(buf[0].p, ints, n_in * sizeof(double), hipMemcpyHostToDevice, st));
+        for (int q = 0; q < rank; ++q) {
+            const int M = (int)rows[q];
+            gemm_f64(true, false, M, n_mo, n_ao, 1.0, buf[q & 1].p, M, dC.p, n_mo, 0.0, nullptr, 0,
+                     buf[(q + 1) & 1].p, n_mo, st);
+        }
+        QBX_HIP(hipMemcpyAsync(out, buf[rank & 1].p, n_out * sizeof(double), hipMemcpyDeviceToHost, st));
+        QBX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int qbx_mo_coulomb_ab(qbx_ctx* ctx, int32_t n_ao, int32_t n1, int32_t n2, const double* eri, const double* C1,
+                      const double* C2, double* Jab)
+{
+    return guarded([&] {
+        if (!ctx) throw QbxError(QBX_EINVAL, "null context");
+        if (n_ao <= 0 || n1 <= 0 || n2 <= 0) throw QbxError(QBX_EINVAL, "n_ao, n1 and n2 must be positive");
+        if (!eri || !C1 || !C2 || !Jab) throw QbxError(QBX_EINVAL, "null array pointer");
+        const size_t nn = (size_t)n_ao * n_ao;
+        if (nn > (size_t)INT32_MAX) throw QbxError(QBX_EUNSUP, "tensor too large for the transform");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        QBX_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        DevBuf<double> dE, dC1, dC2, K1, K2, P, dJ;
+        dE.ensure(nn * nn);
+        QBX_HIP(hipMemcpyAsync(dE.p, eri, nn * nn * sizeof(double), hipMemcpyHostToDevice, st));
+        dC1.upload(std::vector<double>(C1, C1 + (size_t)n_ao * n1), st);
+        dC2.upload(std::vector<double>(C2, C2 + (size_t)n_ao * n2), st);
+        K1.ensure(nn * n1);
+        K2.ensure(nn * n2);
+        pair_columns(dC1.p, n_ao, n1, K1.p, st);
+        pair_columns(dC2.p, n_ao, n2, K2.p, st);
+        // P[m][cd] = sum_ab K1[ab][m] eri[ab][cd];  Jab = P K2
+        P.ensure((size_t)n1 * nn);
+        dJ.ensure((size_t)n1 * n2);
+        const int NN = (int)nn;
+        gemm_f64(true, false, n1, NN, NN, 1.0, K1.p, n1, dE.p, NN, 0.0, nullptr, 0, P.p, NN, st);
+        gemm_f64(false, false, n1, n2, NN, 1.0, P.p, NN, K2.p, n2, 0.0, nullptr, 0, dJ.p, n2, st);
+        QBX_HIP(hipMemcpyAsync(Jab, dJ.p, (size_t)n1 * n2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        QBX_HIP(hipStreamSynchronize(st));
+    });
+}
+
 int32_t qbx_n_classes(void) { return QBX_N_CLASSES; }
 
 int qbx_class_table(int32_t* l4)

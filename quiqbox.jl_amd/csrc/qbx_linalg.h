@@ -11,6 +11,9 @@ namespace qbx {
 void gemm_f64(bool trans_a, bool trans_b, int M, int N, int K, double alpha, const double* A, int lda,
               const double* B, int ldb, double beta, const double* Z, int ldz, double* C, int ldc, hipStream_t s);
 
+// out[(a nao + b) nmo + m] = C[a][m] C[b][m] for the nao x nmo matrix C
+void pair_columns(const double* C, int nao, int nmo, double* out, hipStream_t s);
+
 // Gershgorin interval of each row of the symmetric n x n matrix P:
 // lohi[2i] = P_ii - sum_j!=i |P_ij|, lohi[2i+1] = P_ii + sum_j!=i |P_ij|
 void gershgorin(const double* P, int n, double* lohi, hipStream_t s);

@@ -35,7 +35,10 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmF64 g)
     const int lane = t & 63;
     const int w = t >> 6;
     const int wr = w >> 1, wc = w & 1;
-    const int row0 = blockIdx.y * GT, col0 = blockIdx.x * GT;
+    // 1-D grid, the column tiles of one row panel adjacent: a tall operand
+    // (M up to 2^31 rows) is read from HBM once and reused from L2
+    const int ntile = (g.N + GT - 1) / GT;
+    const int row0 = (int)(blockIdx.x / ntile) * GT, col0 = (int)(blockIdx.x % ntile) * GT;
     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
     for (int k0 = 0; k0 < g.K; k0 += GK) {
         // 512 elements of each tile, two per thread, coalesced along the
@@ -76,6 +79,18 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmF64 g)
             g.C[(size_t)i * g.ldc + j] = v;
         }
     }
+}
+
+// out[(a nao + b) nmo + m] = C[a][m] C[b][m]: the orbital-pair columns of the
+// opposite-spin Coulomb matrix
+__global__ void pair_columns_kernel(const double* C, int nao, int nmo, size_t tot, double* out)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= tot) return;
+    const int m = (int)(e % nmo);
+    const size_t ab = e / nmo;
+    const int a = (int)(ab / nao), b = (int)(ab % nao);
+    out[e] = C[(size_t)a * nmo + m] * C[(size_t)b * nmo + m];
 }
 
 __global__ void gershgorin_kernel(const double* P, int n, double* lohi)
@@ -137,11 +152,22 @@ void gemm_f64(bool trans_a, bool trans_b, int M, int N, int K, double alpha, con
     if (M <= 0 || N <= 0) return;
     if (beta != 0.0 && !Z) throw QbxError(QBX_EINVAL, "GEMM addend missing");
     GemmF64 g{A, B, Z, C, M, N, K, lda, ldb, ldz, ldc, alpha, beta};
-    const dim3 grid((unsigned)((N + GT - 1) / GT), (unsigned)((M + GT - 1) / GT)), block(256);
+    const size_t tiles = (size_t)((N + GT - 1) / GT) * (size_t)((M + GT - 1) / GT);
+    if (tiles > 0x7fffffffu) throw QbxError(QBX_EUNSUP, "GEMM too large for one grid");
+    const dim3 grid((unsigned)tiles), block(256);
     if (!trans_a && !trans_b) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, block, 0, s, g);
     else if (trans_a && !trans_b) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, block, 0, s, g);
     else if (!trans_a && trans_b) hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, block, 0, s, g);
     else hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, block, 0, s, g);
+    QBX_HIP(hipGetLastError());
+}
+
+void pair_columns(const double* C, int nao, int nmo, double* out, hipStream_t s)
+{
+    const size_t tot = (size_t)nao * nao * nmo;
+    if (tot == 0) return;
+    hipLaunchKernelGGL(pair_columns_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, C, nao, nmo, tot,
+                       out);
     QBX_HIP(hipGetLastError());
 }
 

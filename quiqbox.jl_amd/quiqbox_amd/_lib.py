@@ -48,6 +48,8 @@ SIGNATURES = {
     "qbx_basis_shells": (_i32, [_vp, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd]),
     "qbx_rhf_gradient": (_i32, [_vp, _vp, _pd, _pd, _i32, _pd, _pd, _d, _pd, _pd, _pd]),
     "qbx_density_purify": (_i32, [_vp, _i32, _i32, _pd, _pd, _pd, _i32, _pi, _pd]),
+    "qbx_mo_transform": (_i32, [_vp, _i32, _i32, _i32, _pd, _pd, _pd]),
+    "qbx_mo_coulomb_ab": (_i32, [_vp, _i32, _i32, _i32, _pd, _pd, _pd, _pd]),
     "qbx_gemm_f64": (_i32, [_i32, _i32, _i32, _i32, _i32, _d, _vp, _i32, _vp, _i32, _d, _vp, _i32, _vp]),
     "qbx_gemm_bf16": (_i32, [_i32, _i32, _i32, _i32, _i32, ctypes.c_float, _vp, _i32, _vp, _i32, ctypes.c_float,
                              _vp, _i32, _i32, _vp, _vp, _vp]),

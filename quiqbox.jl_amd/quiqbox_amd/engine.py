@@ -69,6 +69,42 @@ class Context:
         return D, int(it.value), float(idem.value)
 
 
+    def mo_transform(self, ints, C):
+        """changeOrbitalBasis (Interface.jl:376-380) of a one-body (2-index)
+        or two-body (4-index) AO tensor with the n_ao x n_mo matrix C: one
+        FP64 matrix-core GEMM per index on the device (qbx_mo_transform)."""
+        A = np.ascontiguousarray(ints, dtype=np.float64)
+        c = np.ascontiguousarray(C, dtype=np.float64)
+        if c.ndim != 2:
+            raise ValueError("C must be a matrix")
+        nao, nmo = c.shape
+        if A.ndim not in (2, 4) or any(d != nao for d in A.shape):
+            raise ValueError("integrals must be n_ao^2 or n_ao^4 with n_ao = C.shape[0]")
+        out = np.empty((nmo,) * A.ndim, dtype=np.float64)
+        pd = ctypes.POINTER(ctypes.c_double)
+        with self.lock:
+            _lib.check(_lib.load().qbx_mo_transform(self.h, A.ndim, nao, nmo, A.ctypes.data_as(pd),
+                                                    c.ctypes.data_as(pd), out.ctypes.data_as(pd)))
+        return out
+
+    def mo_coulomb_ab(self, eri, C1, C2):
+        """getJab (Interface.jl:382-393): Coulomb matrix between the orbitals
+        of C1 and those of C2 (qbx_mo_coulomb_ab)."""
+        E = np.ascontiguousarray(eri, dtype=np.float64)
+        c1 = np.ascontiguousarray(C1, dtype=np.float64)
+        c2 = np.ascontiguousarray(C2, dtype=np.float64)
+        nao = E.shape[0]
+        if E.shape != (nao,) * 4 or c1.ndim != 2 or c2.ndim != 2 or c1.shape[0] != nao or c2.shape[0] != nao:
+            raise ValueError("eri must be n_ao^4 and C1, C2 have n_ao rows")
+        J = np.empty((c1.shape[1], c2.shape[1]), dtype=np.float64)
+        pd = ctypes.POINTER(ctypes.c_double)
+        with self.lock:
+            _lib.check(_lib.load().qbx_mo_coulomb_ab(self.h, nao, c1.shape[1], c2.shape[1], E.ctypes.data_as(pd),
+                                                     c1.ctypes.data_as(pd), c2.ctypes.data_as(pd),
+                                                     J.ctypes.data_as(pd)))
+        return J
+
+
 def default_context():
     global _default_ctx
     with _ctx_lock:

@@ -7,7 +7,7 @@ are bit-identical by contract, Coulomb-test.jl:102,149).
 """
 import numpy as np
 
-from .engine import GPUBasis
+from .engine import GPUBasis, default_context
 from .particles import NuclearCluster, getCharge
 
 
@@ -98,20 +98,15 @@ def elecRepulsion(orb1, orb2, orb3, orb4, lazyCompute=True):
 
 def changeOrbitalBasis(DbodyInt, *C):
     """Transform one- or two-body AO integrals with orbital coefficients
-    (Interface.jl:376-406).  With two coefficient matrices and a 4-index
-    tensor: (ERI_alpha, ERI_beta, J_alphabeta)."""
+    (Interface.jl:376-406) on the GPU, one matrix-core GEMM per index.  With
+    two coefficient matrices and a 4-index tensor: (ERI_alpha, ERI_beta,
+    J_alphabeta)."""
+    ctx = default_context()
     A = np.asarray(DbodyInt)
     if len(C) == 1:
-        c = np.asarray(C[0])
-        if A.ndim == 2:
-            return c.T @ A @ c
-        if A.ndim == 4:
-            return np.einsum("abcd,ai,bj,ck,dl->ijkl", A, c, c, c, c, optimize=True)
-        raise ValueError("one- or two-body integrals expected")
+        if A.ndim not in (2, 4):
+            raise ValueError("one- or two-body integrals expected")
+        return ctx.mo_transform(A, C[0])
     if len(C) == 2 and A.ndim == 4:
-        c1, c2 = (np.asarray(x) for x in C)
-        e1 = changeOrbitalBasis(A, c1)
-        e2 = changeOrbitalBasis(A, c2)
-        jab = np.einsum("abcd,am,bm,cn,dn->mn", A, c1, c1, c2, c2, optimize=True)
-        return e1, e2, jab
+        return ctx.mo_transform(A, C[0]), ctx.mo_transform(A, C[1]), ctx.mo_coulomb_ab(A, C[0], C[1])
     raise ValueError("unsupported changeOrbitalBasis arguments")

@@ -3,6 +3,7 @@ HartreeFock-test.jl literals; error paths of runHartreeFock."""
 import numpy as np
 import pytest
 
+import oracle
 import quiqbox_amd as qb
 from helpers import OracleEngine, literals, molecule_basis
 from quiqbox_amd.systems import h10_sto3g
@@ -41,7 +42,7 @@ def test_hho_rhf_against_reference():
     H = eng.core_hamiltonian(nuc.charges, nuc.coord_array)
     E_ao = qb.getEhf(H, eng.eri(), res.coeff, (5,))
     assert res.energy[0] == pytest.approx(E_ao, abs=1e-10)
-    E_mo = qb.getEhfMO((qb.changeOrbitalBasis(H, C),), (qb.changeOrbitalBasis(eng.eri(), C),), (5,))
+    E_mo = qb.getEhfMO((oracle.change_orbital_basis(H, C),), (oracle.change_orbital_basis(eng.eri(), C),), (5,))
     assert E_mo == pytest.approx(res.energy[0], abs=1e-12)
 
 
@@ -61,8 +62,10 @@ def test_hho_uhf_against_reference():
     for D in res.density:
         np.testing.assert_allclose(D @ S @ D, D, atol=L["tol_E"])
     H = eng.core_hamiltonian(nuc.charges, nuc.coord_array)
-    a, b, c = qb.changeOrbitalBasis(eng.eri(), *res.coeff)
-    E_mo = qb.getEhfMO(tuple(qb.changeOrbitalBasis(H, Cs) for Cs in res.coeff), (a, b), (5, 5), c)
+    E = eng.eri()
+    a, b = (oracle.change_orbital_basis(E, Cs) for Cs in res.coeff)
+    c = oracle.coulomb_ab(E, *res.coeff)
+    E_mo = qb.getEhfMO(tuple(oracle.change_orbital_basis(H, Cs) for Cs in res.coeff), (a, b), (5, 5), c)
     assert E_mo == pytest.approx(res.energy[0], abs=1e-12)
 
 
