@@ -12,7 +12,6 @@
 namespace qbx {
 namespace {
 
-constexpr int GT = 32;   // block tile (GT x GT outputs, four 16 x 16 wave tiles)
 constexpr int GK = 16;   // K step (four MFMA steps)
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -26,59 +25,92 @@ struct GemmF64 {
     double alpha, beta;
 };
 
-template <bool TA, bool TB>
+// Block of four waves in a 2 x 2 arrangement; each wave owns WT x WT tiles of
+// 16 x 16 outputs, so the block tile is 32 WT square.  WT = 2 reads half the
+// LDS bytes per MFMA of WT = 1 and is taken where the grid still has several
+// blocks per CU.
+template <bool TA, bool TB, int WT>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmF64 g)
 {
-    __shared__ double As[GK][GT + 1];   // As[k][i] = op(A)[row0 + i][k0 + k]
-    __shared__ double Bs[GK][GT + 1];   // Bs[k][j] = op(B)[k0 + k][col0 + j]
+    constexpr int BT = 32 * WT;
+    constexpr int LOADS = GK * BT / 256;
+    __shared__ double As[GK][BT + 1];   // As[k][i] = op(A)[row0 + i][k0 + k]
+    __shared__ double Bs[GK][BT + 1];   // Bs[k][j] = op(B)[k0 + k][col0 + j]
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int w = t >> 6;
     const int wr = w >> 1, wc = w & 1;
     // 1-D grid, the column tiles of one row panel adjacent: a tall operand
     // (M up to 2^31 rows) is read from HBM once and reused from L2
-    const int ntile = (g.N + GT - 1) / GT;
-    const int row0 = (int)(blockIdx.x / ntile) * GT, col0 = (int)(blockIdx.x % ntile) * GT;
-    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < g.K; k0 += GK) {
-        // 512 elements of each tile, two per thread, coalesced along the
-        // operand's contiguous index
+    const int ntile = (g.N + BT - 1) / BT;
+    const int row0 = (int)(blockIdx.x / ntile) * BT, col0 = (int)(blockIdx.x % ntile) * BT;
+    dbl4 acc[WT][WT];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+    for (int a = 0; a < WT; ++a)
+#pragma unroll
+        for (int b = 0; b < WT; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+    // tile loads: GK x BT elements of each operand, LOADS per thread,
+    // coalesced along the operand's contiguous index; the next K step's loads
+    // are issued before the current step's MFMAs
+    double ra[LOADS], rb[LOADS];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int h = 0; h < LOADS; ++h) {
             const int e = t + 256 * h;
             int i, k;
-            if (TA) { k = e / GT; i = e % GT; } else { i = e / GK; k = e % GK; }
+            if (TA) { k = e / BT; i = e % BT; } else { i = e / GK; k = e % GK; }
             const int gi = row0 + i, gk = k0 + k;
-            double v = 0.0;
-            if (gi < g.M && gk < g.K) v = TA ? g.A[(size_t)gk * g.lda + gi] : g.A[(size_t)gi * g.lda + gk];
-            As[k][i] = v;
+            ra[h] = (gi < g.M && gk < g.K) ? (TA ? g.A[(size_t)gk * g.lda + gi] : g.A[(size_t)gi * g.lda + gk]) : 0.0;
             int j;
-            if (TB) { j = e / GK; k = e % GK; } else { k = e / GT; j = e % GT; }
-            const int gj = col0 + j;
-            const int gk2 = k0 + k;
-            double u = 0.0;
-            if (gj < g.N && gk2 < g.K) u = TB ? g.B[(size_t)gj * g.ldb + gk2] : g.B[(size_t)gk2 * g.ldb + gj];
-            Bs[k][j] = u;
+            if (TB) { j = e / GK; k = e % GK; } else { k = e / BT; j = e % BT; }
+            const int gj = col0 + j, gk2 = k0 + k;
+            rb[h] = (gj < g.N && gk2 < g.K) ? (TB ? g.B[(size_t)gj * g.ldb + gk2] : g.B[(size_t)gk2 * g.ldb + gj]) : 0.0;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < g.K; k0 += GK) {
+#pragma unroll
+        for (int h = 0; h < LOADS; ++h) {
+            const int e = t + 256 * h;
+            int i, k;
+            if (TA) { k = e / BT; i = e % BT; } else { i = e / GK; k = e % GK; }
+            As[k][i] = ra[h];
+            int j;
+            if (TB) { j = e / GK; k = e % GK; } else { k = e / BT; j = e % BT; }
+            Bs[k][j] = rb[h];
         }
         __syncthreads();
+        if (k0 + GK < g.K) fetch(k0 + GK);
 #pragma unroll
         for (int kk = 0; kk < GK / 4; ++kk) {
-            const double a = As[4 * kk + (lane >> 4)][wr * 16 + (lane & 15)];
-            const double b = Bs[4 * kk + (lane >> 4)][wc * 16 + (lane & 15)];
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            double a[WT], b[WT];
+#pragma unroll
+            for (int m = 0; m < WT; ++m) {
+                a[m] = As[4 * kk + (lane >> 4)][(wr * WT + m) * 16 + (lane & 15)];
+                b[m] = Bs[4 * kk + (lane >> 4)][(wc * WT + m) * 16 + (lane & 15)];
+            }
+#pragma unroll
+            for (int m = 0; m < WT; ++m)
+#pragma unroll
+                for (int n = 0; n < WT; ++n) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[n], acc[m][n], 0, 0, 0);
         }
         __syncthreads();
     }
-    const int j = col0 + wc * 16 + (lane & 15);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int i = row0 + wr * 16 + 4 * r + (lane >> 4);
-        if (i < g.M && j < g.N) {
-            double v = g.alpha * acc[r];
-            if (g.beta != 0.0) v += g.beta * g.Z[(size_t)i * g.ldz + j];
-            g.C[(size_t)i * g.ldc + j] = v;
+    for (int m = 0; m < WT; ++m)
+#pragma unroll
+        for (int n = 0; n < WT; ++n) {
+            const int j = col0 + (wc * WT + n) * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = row0 + (wr * WT + m) * 16 + 4 * r + (lane >> 4);
+                if (i < g.M && j < g.N) {
+                    double v = g.alpha * acc[m][n][r];
+                    if (g.beta != 0.0) v += g.beta * g.Z[(size_t)i * g.ldz + j];
+                    g.C[(size_t)i * g.ldc + j] = v;
+                }
+            }
         }
-    }
 }
 
 // out[(a nao + b) nmo + m] = C[a][m] C[b][m]: the orbital-pair columns of the
@@ -152,13 +184,28 @@ void gemm_f64(bool trans_a, bool trans_b, int M, int N, int K, double alpha, con
     if (M <= 0 || N <= 0) return;
     if (beta != 0.0 && !Z) throw QbxError(QBX_EINVAL, "GEMM addend missing");
     GemmF64 g{A, B, Z, C, M, N, K, lda, ldb, ldz, ldc, alpha, beta};
-    const size_t tiles = (size_t)((N + GT - 1) / GT) * (size_t)((M + GT - 1) / GT);
+    // 64 x 64 blocks where they still give 1024 blocks or more (four per CU),
+    // else 32 x 32 blocks
+    const size_t big = (size_t)((N + 63) / 64) * (size_t)((M + 63) / 64);
+    const int bt = big >= 1024 ? 64 : 32;
+    const size_t tiles = (size_t)((N + bt - 1) / bt) * (size_t)((M + bt - 1) / bt);
     if (tiles > 0x7fffffffu) throw QbxError(QBX_EUNSUP, "GEMM too large for one grid");
     const dim3 grid((unsigned)tiles), block(256);
-    if (!trans_a && !trans_b) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, block, 0, s, g);
-    else if (trans_a && !trans_b) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, block, 0, s, g);
-    else if (!trans_a && trans_b) hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, block, 0, s, g);
-    else hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, block, 0, s, g);
+    auto launch = [&](void (*kern)(GemmF64)) {
+        void* args[] = {&g};
+        QBX_HIP(hipLaunchKernel((const void*)kern, grid, block, args, 0, s));
+    };
+    if (bt == 64) {
+        if (!trans_a && !trans_b) launch(gemm_f64_kernel<false, false, 2>);
+        else if (trans_a && !trans_b) launch(gemm_f64_kernel<true, false, 2>);
+        else if (!trans_a && trans_b) launch(gemm_f64_kernel<false, true, 2>);
+        else launch(gemm_f64_kernel<true, true, 2>);
+    } else {
+        if (!trans_a && !trans_b) launch(gemm_f64_kernel<false, false, 1>);
+        else if (trans_a && !trans_b) launch(gemm_f64_kernel<true, false, 1>);
+        else if (!trans_a && trans_b) launch(gemm_f64_kernel<false, true, 1>);
+        else launch(gemm_f64_kernel<true, true, 1>);
+    }
     QBX_HIP(hipGetLastError());
 }
 

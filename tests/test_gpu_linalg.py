@@ -1,4 +1,5 @@
-"""FP64 matrix-core GEMM (qbx_gemm_f64) against numpy, the purified density
+"""FP64 matrix-core GEMM (qbx_gemm_f64) against numpy (ragged sizes with 32 x 32
+blocks; 2113 x 2049 and 70000 x 100 with 64 x 64 blocks), the purified density
 (qbx_density_purify) against the eigenvector density of the reference's SCF
 step (eigen of X' F X, D = C_occ C_occ^T), and SCF runs with
 densitySolver="purify" against the eigen steps (same energies to 1e-10 Eh)."""
@@ -26,7 +27,8 @@ def _p(t):
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
-@pytest.mark.parametrize("M,N,K", [(37, 53, 29), (64, 64, 64), (1, 70, 5), (131, 17, 200)])
+@pytest.mark.parametrize("M,N,K", [(37, 53, 29), (64, 64, 64), (1, 70, 5), (131, 17, 200),
+                                   (2113, 2049, 45), (70000, 100, 24)])
 def test_gemm_f64_all_transposes_ragged_sizes(ta, tb, M, N, K):
     rng = np.random.default_rng(M * 7 + N + K + 2 * ta + tb)
     A = rng.normal(size=(K, M) if ta else (M, K))
