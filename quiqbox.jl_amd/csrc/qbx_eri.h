@@ -233,11 +233,11 @@ __global__ __launch_bounds__(eri_tpb(EriClass<LA, LB, LC, LD, R>::NACC)) void er
         const int mid = (lo + hi) >> 1;
         if (cpre[mid] <= wave) lo = mid; else hi = mid;
     }
-    const int bi = lo;
-    const int nk = (int)(cst(a.prefix)[bi + 1] - cst(a.prefix)[bi]);
-    const int pos = (wave - cpre[bi]) * 64 + lane;
+    const int tb = lo, bi = tb / a.nbk;
+    const int nk = (int)(cst(a.prefix)[tb + 1] - cst(a.prefix)[tb]);
+    const int pos = (wave - cpre[tb]) * 64 + lane;
     const bool inrange = pos < nk;
-    const int ki = cst(a.koff)[bi] + (inrange ? pos : 0);
+    const int ki = cst(a.koff)[tb] + (inrange ? pos : 0);
 
     const int sA = cst(a.bra.shA)[bi], sB = cst(a.bra.shB)[bi];
     const int sC = a.ket.shA[ki], sD = a.ket.shB[ki];
@@ -600,10 +600,10 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void eri_jk_rows_kernel(EriArgs a)
         const int mid = (lo + hi) >> 1;
         if (spre[mid] <= task) lo = mid; else hi = mid;
     }
-    const int bi = lo;
-    const int sc = task - spre[bi];
-    const int nk = (int)(cst(a.prefix)[bi + 1] - cst(a.prefix)[bi]);
-    const int k0 = cst(a.koff)[bi];
+    const int tb = lo, bi = tb / a.nbk;
+    const int sc = task - spre[tb];
+    const int nk = (int)(cst(a.prefix)[tb + 1] - cst(a.prefix)[tb]);
+    const int k0 = cst(a.koff)[tb];
     const int sA = cst(a.bra.shA)[bi], sB = cst(a.bra.shB)[bi];
     const cptr<DevShell> SA = cst(a.sh + sA);
     const cptr<DevShell> SB = cst(a.sh + sB);
@@ -887,7 +887,7 @@ __device__ __forceinline__ void staged_groups(std::integer_sequence<int, Gs...>,
 // its list.  Evaluates the groups into the wave's workspace slot, then writes
 // the dense tensor or digests J/K (J through the sink, K through `kout`).
 template <int LA, int LB, int LC, int LD, typename R, int MODE, class KOut>
-__device__ __forceinline__ void staged_chunk(const EriArgs& a, int bi, int chunk, int lane, const WsSlot<R>& ws,
+__device__ __forceinline__ void staged_chunk(const EriArgs& a, int tb, int chunk, int lane, const WsSlot<R>& ws,
                                              const double* sb, const KOut& kout)
 {
     using Cls = EriClass<LA, LB, LC, LD, R>;
@@ -895,10 +895,11 @@ __device__ __forceinline__ void staged_chunk(const EriArgs& a, int bi, int chunk
     constexpr int LT = LA + LB + LC + LD;
     constexpr int NCD = NC * ND;
     const size_t N = (size_t)a.N;
-    const int nk = (int)(cst(a.prefix)[bi + 1] - cst(a.prefix)[bi]);
+    const int bi = tb / a.nbk;
+    const int nk = (int)(cst(a.prefix)[tb + 1] - cst(a.prefix)[tb]);
     const int pos = chunk * 64 + lane;
     const bool inrange = pos < nk;
-    const int ki = cst(a.koff)[bi] + (inrange ? pos : 0);
+    const int ki = cst(a.koff)[tb] + (inrange ? pos : 0);
     const int sA = cst(a.bra.shA)[bi], sB = cst(a.bra.shB)[bi];
     const int sC = a.ket.shA[ki], sD = a.ket.shB[ki];
     int kn = inrange ? a.ket.ppn[ki] : 0;
@@ -1123,13 +1124,13 @@ void eri_staged_rows_kernel(EriArgs a)
             const int mid = (lo + hi) >> 1;
             if (spre[mid] <= task) lo = mid; else hi = mid;
         }
-        const int bi = lo;
-        const int sc = task - spre[bi];
-        const int nk = (int)(cst(a.prefix)[bi + 1] - cst(a.prefix)[bi]);
+        const int tb = lo, bi = tb / a.nbk;
+        const int sc = task - spre[tb];
+        const int nk = (int)(cst(a.prefix)[tb + 1] - cst(a.prefix)[tb]);
         for (int r = 0; r < ROW_CHUNKS / W; ++r) {
             const int chunk = (sc * (ROW_CHUNKS / W) + r) * W + wib;
             if (chunk * 64 >= nk) break;
-            staged_chunk<LA, LB, LC, LD, R, MODE_JK>(a, bi, chunk, lane, ws, sb, kout);
+            staged_chunk<LA, LB, LC, LD, R, MODE_JK>(a, tb, chunk, lane, ws, sb, kout);
         }
         __syncthreads();
         const int sA = cst(a.bra.shA)[bi], sB = cst(a.bra.shB)[bi];

@@ -52,6 +52,7 @@ struct EriArgs {
     int ngroup;               // twin groups of this class (0: none)
     int ntwtask;              // their tasks
     int twmax;                // largest group (LDS rows per set)
+    int nbk;                  // ket buckets per bra: task t is bra t / nbk, bucket t % nbk
     int lds_max;              // LDS bytes a block may use on this device
     int64_t total;            // Schwarz mode: number of diagonal pairs
     int nwave;                // J/K and dense modes: number of waves

@@ -133,11 +133,11 @@ __global__ __launch_bounds__(256) void eri_grad_kernel(EriArgs a)
         const int mid = (lo + hi) >> 1;
         if (cpre[mid] <= wave) lo = mid; else hi = mid;
     }
-    const int bi = lo;
-    const int nk = (int)(cst(a.prefix)[bi + 1] - cst(a.prefix)[bi]);
-    const int pos = (wave - cpre[bi]) * 64 + lane;
+    const int tb = lo, bi = tb / a.nbk;
+    const int nk = (int)(cst(a.prefix)[tb + 1] - cst(a.prefix)[tb]);
+    const int pos = (wave - cpre[tb]) * 64 + lane;
     const bool inrange = pos < nk;
-    const int ki = cst(a.koff)[bi] + (inrange ? pos : 0);
+    const int ki = cst(a.koff)[tb] + (inrange ? pos : 0);
     const int sA = cst(a.bra.shA)[bi], sB = cst(a.bra.shB)[bi];
     const int sC = a.ket.shA[ki], sD = a.ket.shB[ki];
     const int kn = inrange ? a.ket.ppn[ki] : 0;

@@ -6,12 +6,15 @@ MO-basis integrals equals the SCF energy (HartreeFock-test.jl:88-91,148-151,
 H2O2 STO-3G), and at n_ao = 130, where the first quarter's GEMM has 130^3 rows
 (more row tiles than one grid dimension of 65535 holds), by an orthogonal
 round trip and single elements contracted on the host."""
+import ctypes
+
 import numpy as np
 import pytest
 
 import oracle
 import quiqbox_amd as qb
 from helpers import literals, molecule_basis
+from quiqbox_amd import _lib
 
 pytestmark = pytest.mark.gpu
 
@@ -81,3 +84,23 @@ def test_n130_round_trip_and_elements():
     err = float(np.abs(back - A).max())
     _say("  round trip max err %.2e" % err)
     assert err < 1e-11
+
+
+def test_argument_errors():
+    """QBX_EINVAL (-1) for a bad rank, an empty basis or a null array; ValueError
+    from the host checks."""
+    ctx = qb.default_context()
+    L = _lib.load()
+    x = np.zeros(16)
+    pd = ctypes.POINTER(ctypes.c_double)
+    p = x.ctypes.data_as(pd)
+    assert L.qbx_mo_transform(ctx.h, 3, 2, 2, p, p, p) == -1      # rank must be 2 or 4
+    assert L.qbx_mo_transform(ctx.h, 4, 0, 2, p, p, p) == -1      # empty basis
+    assert L.qbx_mo_transform(ctx.h, 2, 2, 2, None, p, p) == -1   # null tensor
+    assert L.qbx_mo_coulomb_ab(ctx.h, 2, 0, 2, p, p, p, p) == -1
+    with pytest.raises(ValueError):
+        ctx.mo_transform(np.zeros((3, 3, 3)), np.eye(3))                       # neither one- nor two-body
+    with pytest.raises(ValueError):
+        ctx.mo_transform(np.zeros((4, 4)), np.eye(3))                          # C rows != n_ao
+    with pytest.raises(ValueError):
+        qb.changeOrbitalBasis(np.zeros((3, 3)), np.eye(3), np.eye(3))          # two C only for 4-index

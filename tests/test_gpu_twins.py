@@ -1,7 +1,10 @@
 """Twin bras in the Fock build: cc-pVDZ writes oxygen's 1s and 2s as two
 segmented shells over one set of 9 exponents, so their shell pairs share all
 primitive geometry and the J/K kernels evaluate them as one group (one Boys
-evaluation per primitive quartet, one accumulator set per member).  Checked
+evaluation per primitive quartet, one accumulator set per member).  Ket
+buckets (kets ordered by primitive-pair count, one range per bucket and bra)
+are checked the same way against the plain Schwarz order (QBX_KET_BUCKETS=0):
+the screened quartet counts must agree exactly and G to 1e-12.  Checked
 against the oracle's dense getGcore on one water molecule, and against the same
 build with the groups switched off (QBX_TWINS=0 in a fresh context) on a water
 dimer (cross-molecule groups of four), for RHF, two density sets (UHF) and the
@@ -30,16 +33,21 @@ def _density(n, seed):
     return 0.05 * (A + A.T)
 
 
-def _no_twin_context():
-    old = os.environ.get("QBX_TWINS")
-    os.environ["QBX_TWINS"] = "0"
+def _context(**env):
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         return Context(0)
     finally:
-        if old is None:
-            del os.environ["QBX_TWINS"]
-        else:
-            os.environ["QBX_TWINS"] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
+def _no_twin_context():
+    return _context(QBX_TWINS="0")
 
 
 def test_single_water_ccpvdz_against_oracle():
@@ -84,3 +92,43 @@ def test_water_trimer_scf_twins_on_off():
     _say("  E on %.12f off %.12f" % (r_on.energy[0], r_off.energy[0]))
     assert r_on.converged and r_off.converged
     assert abs(r_on.energy[0] - r_off.energy[0]) < 1e-10
+
+
+@pytest.mark.parametrize("nset,precision,screen", [(1, _lib.QBX_PREC_FP64, 0.0), (2, _lib.QBX_PREC_FP64, 0.0),
+                                                   (1, _lib.QBX_PREC_MIXED, 0.0), (1, _lib.QBX_PREC_FP64, 1e-11)])
+def test_water_dimer_ket_buckets_on_off(nset, precision, screen):
+    nuc, bs = water_box(2, 11, basis="cc-pVDZ")
+    on_ctx, off_ctx = Context(0), _context(QBX_KET_BUCKETS="0")
+    for c in (on_ctx, off_ctx):
+        c.set_density_screening(screen)
+    on = qb.GPUBasis(bs, ctx=on_ctx)
+    off = qb.GPUBasis(bs, ctx=off_ctx)
+    Ds = [_density(on.n, 5 + k) for k in range(nset)]
+    if nset == 1:
+        DJ, DK = 2 * Ds[0], Ds[0]
+    else:
+        DJ = np.stack([Ds[0] + Ds[1]] * 2)
+        DK = np.stack(Ds)
+    _say("water dimer cc-pVDZ: nset %d precision %d screen %g, ket buckets on vs off" % (nset, precision, screen))
+    for thr in (1e-12, 1e-9):
+        s_on, s_off = on.jk_stats(thr), off.jk_stats(thr)
+        for key in s_on:
+            assert np.array_equal(np.asarray(s_on[key]), np.asarray(s_off[key])), key
+        Gon = on.jk(DJ, DK, thr, precision)
+        Goff = off.jk(DJ, DK, thr, precision)
+        err = float(np.abs(Gon - Goff).max())
+        _say("  Schwarz %g: max |G_on - G_off| = %.3e (max |G| %.2e)" % (thr, err, float(np.abs(Goff).max())))
+        assert err < 1e-12 * max(1.0, float(np.abs(Goff).max()))
+
+
+def test_dense_eri_ket_buckets_on_off():
+    """The dense tensor (every quartet, no screening) through the bucketed task
+    lists equals the plain order's, on a water dimer in cc-pVDZ (contraction
+    lengths 1, 3 and 9 within one pair class)."""
+    nuc, bs = water_box(2, 3, basis="cc-pVDZ")
+    on = qb.GPUBasis(bs)
+    off = qb.GPUBasis(bs, ctx=_context(QBX_KET_BUCKETS="0"))
+    E_on, E_off = on.eri_dense(), off.eri_dense()
+    err = float(np.abs(E_on - E_off).max())
+    _say("water dimer cc-pVDZ (%d functions): max |ERI_on - ERI_off| = %.3e" % (on.n, err))
+    assert err < 1e-13
