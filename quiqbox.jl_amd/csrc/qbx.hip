@@ -783,8 +783,12 @@ void ensure_schwarz(qbx_basis* b)
         for (int k = 0; k < n; ++k) qpos[perm[k]] = k;
         // buckets where they make the lanes of a wave markedly more even:
         // busy fraction of the primitive loop over chunks of 64 pairs, in
-        // Schwarz order and within buckets (on 3-primitive s/p/d shells
-        // the counts are nearly uniform and the extra partial waves cost more)
+        // Schwarz order and within buckets.  Measured: 24 H2O in cc-pVDZ
+        // gains 1.9-2.8x on this estimate for its s/p pair classes and its
+        // Fock build drops 80 -> 54 ms; the 592-function cluster of
+        // 3-primitive shells gains at most 1.29x, and bucketing its (pp)
+        // kets at that ratio made (pp|pp) 1.55 -> 2.17 ms (extra partial
+        // waves per bra), so the bar is 1.5x.
         auto busy = [&](const std::vector<int>& order, bool by_bucket) {
             double work = 0.0, span = 0.0;
             int mx = 0, fill = 0, prev = -1;
@@ -808,7 +812,7 @@ void ensure_schwarz(qbx_basis* b)
         std::vector<int> bperm = perm;
         std::stable_sort(bperm.begin(), bperm.end(),
                          [&](int x, int y) { return ket_bucket(P.ppn[x]) < ket_bucket(P.ppn[y]); });
-        const bool bucketed = b->ctx->ket_buckets && busy(bperm, true) >= 1.25 * busy_plain;
+        const bool bucketed = b->ctx->ket_buckets && busy(bperm, true) >= 1.5 * busy_plain;
         if (bucketed) {
             std::unique_ptr<PairClass> pl(new PairClass());
             for (int k = 0; k < n; ++k) {
