@@ -51,6 +51,7 @@ quartet (VRR + Boys seed) and per contracted quartet (HRR) for each class,
 counted from the emitted expressions (add/sub/mul = 1, fma pattern a*b+c
 counted as 2).
 """
+import functools
 import itertools
 import json
 import os
@@ -62,6 +63,12 @@ GRAD_LMAX = 1
 # classes evaluated in groups of bra components through a workspace
 STAGED = {(2, 1, 1, 1), (2, 1, 2, 1), (2, 2, 1, 1), (2, 2, 2, 0), (2, 2, 2, 1), (2, 2, 2, 2)}
 STAGE_ACC = 32     # accumulators per group
+COST_PRIM0 = 80    # launch-order cost model: geometry and Boys values per primitive quartet
+COST_DIGEST = 8    # launch-order cost model: J/K digest per Cartesian quartet
+
+
+def ncart_prod(c):
+    return functools.reduce(lambda x, l: x * (l + 1) * (l + 2) // 2, c, 1)
 # Classes whose staged kernel runs at one wave per SIMD with larger groups: the
 # register file of a lone wave (VGPRs plus AGPRs as spill space) holds 48
 # accumulators, so the class makes 21 passes over its primitive quartets
@@ -727,6 +734,17 @@ def main(outdir):
              "// classes with a twin-bra kernel (not staged, at most %d accumulators)" % TWIN_MAX_ACC,
              "static const int qbx_class_twin[%d] = {%s};" % (
                  len(classes), ", ".join("1" if t else "0" for t in twin_ok)),
+             "// launch-order cost model: per primitive shell quartet (geometry and Boys"
+             " values, about %d operations, plus the executed VRR) and per contracted"
+             " shell quartet (executed HRR plus %d per Cartesian quartet for the digest)" % (COST_PRIM0, COST_DIGEST),
+             "static const double qbx_class_cost_prim[%d] = {%s};" % (
+                 len(classes), ", ".join(str(COST_PRIM0 + flops[t].get("vrr_executed_per_prim_quartet",
+                                                                      flops[t]["vrr_per_prim_quartet"]))
+                                         for t in ["%d%d%d%d" % c for c in classes])),
+             "static const double qbx_class_cost_quartet[%d] = {%s};" % (
+                 len(classes), ", ".join(str(flops[t].get("hrr_executed_per_quartet", flops[t]["hrr_per_quartet"])
+                                             + COST_DIGEST * ncart_prod(c))
+                                         for t, c in zip(["%d%d%d%d" % c for c in classes], classes))),
              "#define QBX_FOR_EACH_CLASS(X) \\"]
     lines += ["    X(%d, %d, %d, %d) \\" % c for c in classes]
     lines.append("")

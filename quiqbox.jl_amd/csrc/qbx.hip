@@ -188,6 +188,7 @@ struct qbx_basis {
     double task_thr = -1.0, task_cut = -1.0;
     int task_mode = -1;
     TaskSet t64[QBX_N_CLASSES], t32[QBX_N_CLASSES];
+    double class_cost[QBX_N_CLASSES] = {};   // estimated work of each class (launch order)
     // twin bras: groups of positions in each (Schwarz-sorted) pair class; for
     // the J/K build of a class with groups, the other bras' task lists and
     // the groups' own
@@ -1009,6 +1010,24 @@ void twin_tasks(const qbx_basis* b, int cls, double thr, int mode, double cut, T
     w64.twmax = twmax;
 }
 
+// Estimated work of a class's task lists: primitive shell quartets and
+// contracted shell quartets weighted by the class's cost per unit.
+double class_cost(const qbx_basis* b, int cls, const TaskCounts& tc)
+{
+    const PairClass& PX = pair_view(b, cls, 0);
+    const PairClass& PY = pair_view(b, cls, 1);
+    std::vector<int64_t> kp(PY.n() + 1, 0);
+    for (int j = 0; j < PY.n(); ++j) kp[j + 1] = kp[j] + PY.ppn[j];
+    double prim = 0.0, quart = 0.0;
+    for (int t = 0; t < tc.ntb; ++t) {
+        const int n64 = (int)(tc.p64[t + 1] - tc.p64[t]), n32 = (int)(tc.p32[t + 1] - tc.p32[t]);
+        const int64_t kets = (kp[tc.k64[t] + n64] - kp[tc.k64[t]]) + (kp[tc.k32[t] + n32] - kp[tc.k32[t]]);
+        prim += (double)PX.ppn[t / PY.nbk] * (double)kets;
+        quart += n64 + n32;
+    }
+    return prim * qbx_class_cost_prim[cls] + quart * qbx_class_cost_quartet[cls];
+}
+
 void ensure_tasks(qbx_basis* b, double thr, int mode, double cut)
 {
     if (mode != QBX_PREC_MIXED) cut = 0.0;
@@ -1020,6 +1039,7 @@ void ensure_tasks(qbx_basis* b, double thr, int mode, double cut)
         const int X = class_pc(cls, 0);
         const int nx = b->pc[X].n();
         upload_tasks(b->t64[cls], b->t32[cls], tc, st);
+        b->class_cost[cls] = class_cost(b, cls, tc);
         b->use_twin[cls] = false;
         // only where bra and ket classes differ: within one pair class a bra's
         // kets end at its own Schwarz-sorted position, so the members' common
@@ -1160,10 +1180,11 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         // staged classes first: few long-running waves each, they overlap with
-        // the wide low-L kernels instead of running alone at the end
+        // the wide low-L kernels instead of running alone at the end; then the
+        // largest estimated work first, so the short classes fill the tail
         const bool sx = qbx_class_ws[x] > 0, sy = qbx_class_ws[y] > 0;
         if (sx != sy) return sx;
-        return b->t64[x].total + b->t32[x].total > b->t64[y].total + b->t32[y].total;
+        return b->class_cost[x] > b->class_cost[y];
     });
     for (int oi = 0; oi < QBX_N_CLASSES; ++oi) {
         const int cls = order[oi];
