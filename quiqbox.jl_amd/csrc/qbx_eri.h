@@ -59,6 +59,18 @@ struct JKSink {
 // copies fit a CU) serves more waves.
 __host__ __device__ constexpr int eri_tpb(int nacc) { return nacc <= 3 ? 1024 : nacc <= 18 ? 512 : 256; }
 
+// Boys table extent per order.  Beyond T = 36 (F_0) and T = 40 (F_1) the
+// asymptotic forms sqrt(pi/4T) and F_0/2T are exact to double precision
+// (relative error 2.2e-17 for F_0 at T = 36 and 3.1e-17 for F_1 at T = 40
+// against 40-digit values, falling as e^{-T} beyond), so the classes of total
+// angular momentum 0 and 1 stage only
+// 146 and 162 of the table's rows, 14-15 KB of LDS instead of 23 KB: more
+// resident blocks to hide the latency of the lookup.
+constexpr double boys_tmax(int L) { return L == 0 ? 36.0 : L == 1 ? 40.0 : QBX_BOYS_TMAX; }
+constexpr int boys_rows(int L) { return L == 0 ? 146 : L == 1 ? 162 : QBX_BOYSL_ROWS; }
+static_assert(boys_rows(0) > 36.0 * QBX_BOYSL_INVDT + 0.5 && boys_rows(1) > 40.0 * QBX_BOYSL_INVDT + 0.5,
+              "staged Boys rows must cover the lookup range");
+
 // Boys F_0..F_L from the LDS rows of order L (see qbx_device.h for the table):
 // an 11-term Taylor expansion of F_L about the nearest grid point, e^{-T} from
 // e^{-T_i} times the Taylor series of e^{T_i - T} (|T_i - T| <= 1/8, terms to
@@ -67,7 +79,7 @@ __host__ __device__ constexpr int eri_tpb(int nacc) { return nacc <= 3 ? 1024 : 
 template <int L>
 __device__ __forceinline__ void boys_lds(double T, double* __restrict__ F, const double* __restrict__ sb)
 {
-    if (T < QBX_BOYS_TMAX) {
+    if (T < boys_tmax(L)) {
         const int i = __double2int_rn(T * QBX_BOYSL_INVDT);
         const double md = (double)i * QBX_BOYSL_DT - T;
         const double2* row = (const double2*)(sb + i * QBX_BOYSL_STRIDE);
@@ -110,11 +122,11 @@ __device__ __forceinline__ void boys_lds(double T, double* __restrict__ F, const
 }
 
 // Copy the Boys rows of the class's total angular momentum into LDS.
-template <int TPB>
+template <int TPB, int ROWS = QBX_BOYSL_ROWS>
 __device__ __forceinline__ void stage_boys(double2* __restrict__ dst, const double* __restrict__ src)
 {
     const double2* s = (const double2*)src;
-    for (int i = threadIdx.x; i < QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2; i += TPB) dst[i] = s[i];
+    for (int i = threadIdx.x; i < ROWS * QBX_BOYSL_STRIDE / 2; i += TPB) dst[i] = s[i];
 }
 
 // 1/sqrt(x) for finite x > 0 (p + q of a primitive quartet): the hardware
@@ -218,8 +230,8 @@ __global__ __launch_bounds__(eri_tpb(EriClass<LA, LB, LC, LD, R>::NACC)) void er
     constexpr int LT = LA + LB + LC + LD;
     constexpr int NCD = NC * ND;
     constexpr bool PREFETCH = Cls::NACC <= 18;
-    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
-    stage_boys<TPB>(sboys, a.boysl);
+    __shared__ double2 sboys[boys_rows(LT) * QBX_BOYSL_STRIDE / 2];
+    stage_boys<TPB, boys_rows(LT)>(sboys, a.boysl);
     __syncthreads();
     const double* sb = (const double*)sboys;
 
@@ -580,12 +592,12 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void eri_jk_rows_kernel(EriArgs a)
     constexpr int NCD = NC * ND;
     constexpr int NROW = NA + NB;
     constexpr bool PREFETCH = Cls::NACC <= 18;
-    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
+    __shared__ double2 sboys[boys_rows(LT) * QBX_BOYSL_STRIDE / 2];
     extern __shared__ double krow[];    // [nset][NROW][N]
     const int Ni = a.N;
     const size_t N = (size_t)a.N;
     const int nrowel = a.nset * NROW * Ni;
-    stage_boys<TPB>(sboys, a.boysl);
+    stage_boys<TPB, boys_rows(LT)>(sboys, a.boysl);
     for (int i = threadIdx.x; i < nrowel; i += TPB) krow[i] = 0.0;
     __syncthreads();
     const double* sb = (const double*)sboys;
@@ -676,12 +688,12 @@ __global__ __launch_bounds__(64 * TWIN_WAVES) void eri_jk_twin_kernel(EriArgs a)
     constexpr int NROW = NA + NB;
     constexpr int TW = TWIN_MAX;
     static_assert(ROW_CHUNKS % TWIN_WAVES == 0, "chunks per task must split evenly over the waves");
-    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
+    __shared__ double2 sboys[boys_rows(LT) * QBX_BOYSL_STRIDE / 2];
     extern __shared__ double krow[];    // [nset][twmax][NROW][N]
     const int Ni = a.N;
     const int tw = a.twmax;
     const int nrowel = a.nset * tw * NROW * Ni;
-    stage_boys<TPB>(sboys, a.boysl);
+    stage_boys<TPB, boys_rows(LT)>(sboys, a.boysl);
     for (int i = threadIdx.x; i < nrowel; i += TPB) krow[i] = 0.0;
     __syncthreads();
     const double* sb = (const double*)sboys;
@@ -1147,8 +1159,8 @@ __global__ __launch_bounds__(256) void eri_schwarz_kernel(EriArgs a)
     constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB);
     constexpr int LT = 2 * (LA + LB);
     constexpr int NCD = NA * NB;
-    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
-    stage_boys<256>(sboys, a.boysl);
+    __shared__ double2 sboys[boys_rows(LT) * QBX_BOYSL_STRIDE / 2];
+    stage_boys<256, boys_rows(LT)>(sboys, a.boysl);
     __syncthreads();
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.total) return;

@@ -119,8 +119,8 @@ __global__ __launch_bounds__(256) void eri_grad_kernel(EriArgs a)
     constexpr int LT = LA + LB + LC + LD;
     constexpr int NACC = Adj::NACC;
     constexpr int NINT = Adj::NINT;
-    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
-    stage_boys<256>(sboys, a.boysl);    // rows of order LT + 1
+    __shared__ double2 sboys[boys_rows(LT + 1) * QBX_BOYSL_STRIDE / 2];
+    stage_boys<256, boys_rows(LT + 1)>(sboys, a.boysl);    // rows of order LT + 1
     __syncthreads();
     const double* sb = (const double*)sboys;
 
