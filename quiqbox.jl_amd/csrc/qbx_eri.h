@@ -59,17 +59,19 @@ struct JKSink {
 // copies fit a CU) serves more waves.
 __host__ __device__ constexpr int eri_tpb(int nacc) { return nacc <= 3 ? 1024 : nacc <= 18 ? 512 : 256; }
 
-// Boys table extent per order.  Beyond T = 36 (F_0) and T = 40 (F_1) the
-// asymptotic forms sqrt(pi/4T) and F_0/2T are exact to double precision
-// (relative error 2.2e-17 for F_0 at T = 36 and 3.1e-17 for F_1 at T = 40
-// against 40-digit values, falling as e^{-T} beyond), so the classes of total
-// angular momentum 0 and 1 stage only
-// 146 and 162 of the table's rows, 14-15 KB of LDS instead of 23 KB: more
+// Boys table extent per order.  Beyond T_L = 36, 40, 44, 47, 50, 52, 55, 57
+// and 60 (L = 0..8) the asymptotic forms F_0 = sqrt(pi/4T), F_{m+1} =
+// F_m (2m+1)/2T are exact to double precision for every order m <= L
+// (relative error below 4e-17 from the first grid point past 35.5, 39.75,
+// 43.25, 46.25, 49, 51.75, 54.25, 56.75 and 59 against 40-digit values,
+// falling as e^{-T} beyond), so a kernel of total angular momentum L stages
+// only the rows below T_L: 14 KB of LDS instead of 23 KB at L = 0, more
 // resident blocks to hide the latency of the lookup.
-constexpr double boys_tmax(int L) { return L == 0 ? 36.0 : L == 1 ? 40.0 : QBX_BOYS_TMAX; }
-constexpr int boys_rows(int L) { return L == 0 ? 146 : L == 1 ? 162 : QBX_BOYSL_ROWS; }
-static_assert(boys_rows(0) > 36.0 * QBX_BOYSL_INVDT + 0.5 && boys_rows(1) > 40.0 * QBX_BOYSL_INVDT + 0.5,
-              "staged Boys rows must cover the lookup range");
+constexpr double kBoysTmax[QBX_BOYSL_LMAX + 1] = {36.0, 40.0, 44.0, 47.0, 50.0, 52.0, 55.0, 57.0, 60.0, 60.0};
+constexpr double boys_tmax(int L) { return kBoysTmax[L]; }
+constexpr int boys_rows(int L) { return (int)(kBoysTmax[L] * QBX_BOYSL_INVDT) + 2; }
+static_assert(kBoysTmax[QBX_BOYSL_LMAX] <= QBX_BOYS_TMAX && boys_rows(QBX_BOYSL_LMAX) <= QBX_BOYSL_ROWS,
+              "the per-order extent must stay inside the table");
 
 // Boys F_0..F_L from the LDS rows of order L (see qbx_device.h for the table):
 // an 11-term Taylor expansion of F_L about the nearest grid point, e^{-T} from
