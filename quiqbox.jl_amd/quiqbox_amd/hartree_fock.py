@@ -308,8 +308,10 @@ def rmsOf(a):
 
 
 def getEresidual(F, D, S):
-    """FDS - SDF (HartreeFock.jl:1298-1299)"""
-    return F @ D @ S - S @ D @ F
+    """FDS - SDF (HartreeFock.jl:1298-1299).  F, D and S are symmetric, so
+    SDF = (FDS)^T: two products instead of four."""
+    M = F @ D @ S
+    return M - M.T
 
 
 # ---------------------------------------------------------------------------
