@@ -921,6 +921,7 @@ void task_counts(const qbx_basis* b, int cls, double thr, int mode, double cut, 
     const PairClass& PY = pair_view(b, cls, 1);
     const int nk = PY.nbk;
     const int nt = nx * nk;
+    const int rc = qbx_class_ws[cls] > 0 ? STAGED_ROW_CHUNKS : ROW_CHUNKS;   // chunks per K-row task
     tc.ntb = nt;
     tc.p64.assign(nt + 1, 0);
     tc.p32.assign(nt + 1, 0);
@@ -943,8 +944,8 @@ void task_counts(const qbx_basis* b, int cls, double thr, int mode, double cut, 
             tc.p32[t + 1] = tc.p32[t] + (cnt - c64);
             tc.w64[t + 1] = tc.w64[t] + (c64 - k0 + 63) / 64;
             tc.w32[t + 1] = tc.w32[t] + (cnt - c64 + 63) / 64;
-            tc.s64[t + 1] = tc.s64[t] + ((c64 - k0 + 63) / 64 + ROW_CHUNKS - 1) / ROW_CHUNKS;
-            tc.s32[t + 1] = tc.s32[t] + ((cnt - c64 + 63) / 64 + ROW_CHUNKS - 1) / ROW_CHUNKS;
+            tc.s64[t + 1] = tc.s64[t] + ((c64 - k0 + 63) / 64 + rc - 1) / rc;
+            tc.s32[t + 1] = tc.s32[t] + ((cnt - c64 + 63) / 64 + rc - 1) / rc;
         }
 }
 

@@ -1101,7 +1101,7 @@ void eri_staged_kernel(EriArgs a)
 }
 
 // Staged J/K with the bra's K rows in LDS: a persistent grid of blocks, each
-// taking one (bra, up to ROW_CHUNKS chunks) task at a time; its waves share
+// taking one (bra, up to STAGED_ROW_CHUNKS chunks) task at a time; its waves share
 // the task's chunks, K goes to the rows, and the rows are flushed to K once
 // per task.
 template <int LA, int LB, int LC, int LD, typename R>
@@ -1115,7 +1115,7 @@ void eri_staged_rows_kernel(EriArgs a)
     constexpr int NROW = NA + NB;
     constexpr int NCD = qbx_ncart(LC) * qbx_ncart(LD);
     constexpr int WSV = Cls::NE * NCD;
-    static_assert(ROW_CHUNKS % W == 0, "chunks per task must split evenly over the waves");
+    static_assert(STAGED_ROW_CHUNKS % W == 0, "chunks per task must split evenly over the waves");
     __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
     extern __shared__ double krow[];    // [nset][NROW][N]
     const int Ni = a.N;
@@ -1141,8 +1141,8 @@ void eri_staged_rows_kernel(EriArgs a)
         const int tb = lo, bi = tb / a.nbk;
         const int sc = task - spre[tb];
         const int nk = (int)(cst(a.prefix)[tb + 1] - cst(a.prefix)[tb]);
-        for (int r = 0; r < ROW_CHUNKS / W; ++r) {
-            const int chunk = (sc * (ROW_CHUNKS / W) + r) * W + wib;
+        for (int r = 0; r < STAGED_ROW_CHUNKS / W; ++r) {
+            const int chunk = (sc * (STAGED_ROW_CHUNKS / W) + r) * W + wib;
             if (chunk * 64 >= nk) break;
             staged_chunk<LA, LB, LC, LD, R, MODE_JK>(a, tb, chunk, lane, ws, sb, kout);
         }

@@ -15,6 +15,10 @@ enum EriMode { MODE_JK = 0, MODE_DENSE = 1, MODE_SCHWARZ = 2 };
 #endif
 constexpr int ROW_WAVES = QBX_ROW_WAVES;
 constexpr int ROW_CHUNKS = 16;
+// Chunks per task of the staged K-row kernel: its waves are long, and smaller
+// tasks balance better (592-function cluster: (dp|pp) 5.05 -> 4.33 ms, build
+// 41.0 -> 40.1 ms; the low-L K-row kernels keep 16, water's (ps|ss) lost 4% at 8).
+constexpr int STAGED_ROW_CHUNKS = 8;
 constexpr int ROW_LDS_MAX = 64 * 1024;
 // Twin bras: shell pairs of one pair class with the same primitive geometry
 // and primitive pairs, differing only in contraction coefficients (shells
