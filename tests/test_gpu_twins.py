@@ -8,7 +8,9 @@ the screened quartet counts must agree exactly and G to 1e-12.  Checked
 against the oracle's dense getGcore on one water molecule, and against the same
 build with the groups switched off (QBX_TWINS=0 in a fresh context) on a water
 dimer (cross-molecule groups of four), for RHF, two density sets (UHF) and the
-mixed-precision task split, and through a converged SCF."""
+mixed-precision task split, and through a converged SCF.  A water dimer with
+everything on (ket buckets, twin bras, the staged classes' plain-order ket
+lists) is checked against the oracle's dense getGcore as well."""
 import os
 
 import numpy as np
@@ -132,3 +134,24 @@ def test_dense_eri_ket_buckets_on_off():
     err = float(np.abs(E_on - E_off).max())
     _say("water dimer cc-pVDZ (%d functions): max |ERI_on - ERI_off| = %.3e" % (on.n, err))
     assert err < 1e-13
+
+
+def test_water_dimer_ccpvdz_against_oracle():
+    """Ket buckets, twin bras and the staged classes' plain-order ket lists all
+    active (50 functions, d shells on oxygen) against the oracle's dense
+    getGcore, RHF and one UHF pair."""
+    nuc, bs = water_box(2, 11, basis="cc-pVDZ")
+    gb = qb.GPUBasis(bs)
+    ob = oracle.Basis(gb.flat.as_dict())
+    E = ob.eri_dense()
+    Da, Db = _density(gb.n, 21), _density(gb.n, 22)
+    _say("water dimer cc-pVDZ (%d functions): GPU J/K vs oracle getGcore" % gb.n)
+    G = gb.jk(2 * Da, Da, 0.0)
+    err = float(np.abs(G - oracle.gcore(E, 2 * Da, Da)).max())
+    _say("  RHF max |G - G_oracle| = %.3e" % err)
+    assert err < 1e-11
+    Gu = gb.jk(np.stack([Da + Db] * 2), np.stack([Da, Db]), 0.0)
+    for s, Ds in enumerate((Da, Db)):
+        err = float(np.abs(Gu[s] - oracle.gcore(E, Da + Db, Ds)).max())
+        _say("  UHF set %d max |G - G_oracle| = %.3e" % (s, err))
+        assert err < 1e-11
