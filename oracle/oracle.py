@@ -26,9 +26,10 @@ class _Basis(ctypes.Structure):
                 ("n_bf", _i32), ("bf_ptr", _pi), ("bf_prim", _pi), ("bf_w", _pd)]
 
 
-def build():
-    """Compile liboracle.so with the committed Makefile."""
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+def build(force=False):
+    """Compile liboracle.so with the committed Makefile (force: even when it is
+    up to date)."""
+    subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", _HERE], check=True)
 
 
 def lib():

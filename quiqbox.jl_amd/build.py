@@ -49,11 +49,11 @@ def _digest(paths, extra):
     return h.hexdigest()[:16]
 
 
-def _compile(src, extra, headers, verbose):
+def _compile(src, extra, headers, verbose, force):
     os.makedirs(OBJ, exist_ok=True)
     tag = _digest([src] + headers, extra)
     obj = os.path.join(OBJ, os.path.basename(src).replace(".hip", "") + "." + tag + ".o")
-    if os.path.exists(obj):
+    if os.path.exists(obj) and not force:
         return obj, 0.0, ""
     t0 = time.time()
     cmd = [HIPCC] + FLAGS + list(extra) + ["-c", src, "-o", obj + ".tmp"]
@@ -65,7 +65,10 @@ def _compile(src, extra, headers, verbose):
     return obj, time.time() - t0, msg
 
 
-def build(verbose=True, extra=(), jobs=None):
+def build(verbose=True, extra=(), jobs=None, force=False):
+    """Generate the ERI sources and link libqbx_hip.so.  Objects are cached by
+    a digest of their sources, headers and flags; force=True compiles every
+    source regardless."""
     t0 = time.time()
     generate()
     headers = sorted(glob.glob(os.path.join(CSRC, "*.h")) + [os.path.join(GEN, "eri_table.inc"),
@@ -76,7 +79,7 @@ def build(verbose=True, extra=(), jobs=None):
     jobs = jobs or max(1, min(len(srcs), os.cpu_count() or 4))
     objs = {}
     with cf.ThreadPoolExecutor(jobs) as ex:
-        futs = {ex.submit(_compile, s, extra, headers, verbose): s for s in srcs}
+        futs = {ex.submit(_compile, s, extra, headers, verbose, force): s for s in srcs}
         for fu in cf.as_completed(futs):
             obj, dt, msg = fu.result()
             objs[futs[fu]] = obj
