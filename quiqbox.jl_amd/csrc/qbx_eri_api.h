@@ -14,10 +14,13 @@ enum EriMode { MODE_JK = 0, MODE_DENSE = 1, MODE_SCHWARZ = 2 };
 #define QBX_ROW_WAVES 4
 #endif
 constexpr int ROW_WAVES = QBX_ROW_WAVES;
-constexpr int ROW_CHUNKS = 16;
+// Chunks per task of the low-L K-row and twin kernels: fewer, larger tasks
+// (fewer K-row flushes) measured 50.9 -> 49.6 ms per Fock build of 24 H2O in
+// cc-pVDZ at 32 instead of 16, the 592-function cluster unchanged.
+constexpr int ROW_CHUNKS = 32;
 // Chunks per task of the staged K-row kernel: its waves are long, and smaller
 // tasks balance better (592-function cluster: (dp|pp) 5.05 -> 4.33 ms, build
-// 41.0 -> 40.1 ms; the low-L K-row kernels keep 16, water's (ps|ss) lost 4% at 8).
+// 41.0 -> 40.1 ms against 16; water's low-L classes lost 4% at 8).
 constexpr int STAGED_ROW_CHUNKS = 8;
 constexpr int ROW_LDS_MAX = 64 * 1024;
 // Twin bras: shell pairs of one pair class with the same primitive geometry
