@@ -16,7 +16,8 @@ enum EriMode { MODE_JK = 0, MODE_DENSE = 1, MODE_SCHWARZ = 2 };
 constexpr int ROW_WAVES = QBX_ROW_WAVES;
 // Chunks per task of the low-L K-row and twin kernels: fewer, larger tasks
 // (fewer K-row flushes) measured 50.9 -> 49.6 ms per Fock build of 24 H2O in
-// cc-pVDZ at 32 instead of 16, the 592-function cluster unchanged.
+// cc-pVDZ at 32 instead of 16, the 592-function cluster unchanged; at 64 the
+// cluster lost (38.6 -> 39.4 ms).
 constexpr int ROW_CHUNKS = 32;
 // Chunks per task of the staged K-row kernel: its waves are long, and smaller
 // tasks balance better (592-function cluster: (dp|pp) 5.05 -> 4.33 ms, build
