@@ -121,7 +121,9 @@ int  qbx_mo_coulomb_ab(qbx_ctx* ctx, int32_t n_ao, int32_t n1, int32_t n2, const
 int  qbx_jk(qbx_ctx* ctx, qbx_basis* basis, int32_t n_set, const double* DJ,
             const double* DK, double* G, double schwarz_thr, int32_t precision_mode);
 /* Same with device pointers; enqueued on `stream` (a hipStream_t, NULL for the
-   context stream) and returning without synchronizing. */
+   context stream) and returning without synchronizing.  Builds of one basis
+   share its work buffers: a call on another stream than the basis's last
+   build waits on the host for that build to finish first. */
 int  qbx_jk_device(qbx_ctx* ctx, qbx_basis* basis, int32_t n_set, const double* dDJ,
                    const double* dDK, double* dG, double schwarz_thr,
                    int32_t precision_mode, void* stream);
@@ -136,10 +138,24 @@ int  qbx_class_table(int32_t* l4 /* 4*n_classes */);
 int  qbx_jk_stats(qbx_ctx* ctx, qbx_basis* basis, double schwarz_thr,
                   double fp64_cut, int64_t* shell_quartets, int64_t* cart_quartets,
                   int64_t* prim_quartets, int64_t* fp64_shell_quartets);
-/* When enabled, qbx_jk records per-class kernel time (HIP events on the
-   launching stream); qbx_last_timing writes n_classes milliseconds. */
-int  qbx_set_timing(qbx_ctx* ctx, int32_t enable);
+/* Per-class kernel time of qbx_jk / qbx_jk_device, from HIP events on the
+   stream each class is launched on.  mode 1: the class kernels of a build run
+   one after another on one stream, and qbx_last_timing writes the last
+   build's n_classes milliseconds and its total.  mode 2: the build runs
+   concurrently as it does untimed (classes spread over the auxiliary
+   streams); every build since the last read (or since qbx_set_timing) is
+   recorded, and qbx_last_timing writes each class's average duration in the
+   concurrent builds and the average build time, then starts a new average.
+   mode 0: off. */
+int  qbx_set_timing(qbx_ctx* ctx, int32_t mode);
 int  qbx_last_timing(qbx_ctx* ctx, float* ms_per_class, float* ms_total);
+
+/* The Boys function the ERI kernels use, evaluated on its own for tests (no
+   reference counterpart; the reference's is computeBoysSequence,
+   BoysFunction.jl:67-77): F_0..F_order at the n arguments T (host arrays,
+   0 <= T <= 1e6), from the same order-`order` table rows staged in LDS, into F
+   (n x (order+1), row-major).  order in [0, 8]. */
+int  qbx_boys_device(qbx_ctx* ctx, int32_t order, int32_t n, const double* T, double* F);
 
 /* Mixed-precision cut: quartets whose Schwarz bound exceeds `cut` run in FP64
    under QBX_PREC_MIXED (default 1e-5). */

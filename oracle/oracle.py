@@ -32,6 +32,19 @@ def build(force=False):
     subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", _HERE], check=True)
 
 
+def use_native_build(out_dir):
+    """Compile the oracle for the host CPU it runs on (-march=native, OpenMP)
+    into out_dir and load that copy instead of liboracle.so.  Used by the CPU
+    baseline timing only: the committed build stays portable."""
+    global _lib, _LIB_PATH
+    path = os.path.join(out_dir, "liboracle_native.so")
+    subprocess.run(["gcc", "-O3", "-fPIC", "-fopenmp", "-ffp-contract=off", "-march=native", "-std=c11",
+                    "-D_GNU_SOURCE", "-shared", "-o", path, os.path.join(_HERE, "qbx_oracle.c"), "-lm"], check=True)
+    _LIB_PATH = path
+    _lib = None
+    return path
+
+
 def lib():
     global _lib
     if _lib is None:

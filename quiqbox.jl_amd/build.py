@@ -1,7 +1,8 @@
 """Build libqbx_hip.so for gfx950 in-tree.
 
 Steps: run csrc/gen_eri.py (one translation unit per ERI angular class into
-csrc/gen/), compile csrc/qbx.hip, csrc/qbx_mlp.hip, csrc/qbx_linalg.hip and every csrc/gen/eri_*.hip to objects in
+csrc/gen/), compile csrc/qbx.hip, csrc/qbx_mlp.hip, csrc/qbx_linalg.hip, csrc/qbx_boys_probe.hip and every
+csrc/gen/eri_*.hip to objects in
 parallel with hipcc, link quiqbox_amd/libqbx_hip.so.  Objects whose sources and
 headers are unchanged are reused.
 """
@@ -74,7 +75,7 @@ def build(verbose=True, extra=(), jobs=None, force=False):
     headers = sorted(glob.glob(os.path.join(CSRC, "*.h")) + [os.path.join(GEN, "eri_table.inc"),
                                                               os.path.join(INCLUDE, "qbx.h")])
     srcs = [os.path.join(CSRC, "qbx.hip"), os.path.join(CSRC, "qbx_mlp.hip"),
-            os.path.join(CSRC, "qbx_linalg.hip")] + sorted(glob.glob(os.path.join(GEN, "eri_*.hip")),
+            os.path.join(CSRC, "qbx_linalg.hip"), os.path.join(CSRC, "qbx_boys_probe.hip")] + sorted(glob.glob(os.path.join(GEN, "eri_*.hip")),
                                                      key=lambda p: -os.path.getsize(p))
     jobs = jobs or max(1, min(len(srcs), os.cpu_count() or 4))
     objs = {}

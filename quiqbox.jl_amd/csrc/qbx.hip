@@ -98,6 +98,7 @@ constexpr double QBX_PRIM_PAIR_CUT = 1e-20;
 // ket range per bucket.  Estimated on the (ss|ss) kets of 24 H2O in cc-pVDZ:
 // lanes busy 36% -> 78% of the primitive loop, 3.5% more waves.
 constexpr int kKetBuckets = 5;
+
 inline int ket_bucket(int ppn) { return ppn <= 1 ? 0 : ppn <= 4 ? 1 : ppn <= 9 ? 2 : ppn <= 27 ? 3 : 4; }
 
 }  // namespace
@@ -112,9 +113,21 @@ struct qbx_ctx {
     hipEvent_t ev_fork = nullptr, ev_join[kStreams] = {};
     DevBuf<double> boys, bexp;
     DevBuf<double> boysl;          // per-order rows for the ERI kernels (LDS-staged)
-    bool timing = false;
+    int timing = 0;                // 0 off, 1 serialized per-class timing, 2 concurrent per-class timing
     std::vector<float> last_ms;
     float last_total = 0.f;
+    // concurrent timing: a ring of event sets, one per Fock build, read back
+    // (and summed) when a set is reused or at qbx_last_timing
+    struct TimingSet {
+        hipEvent_t beg_l[QBX_N_CLASSES], end_l[QBX_N_CLASSES], beg_s[QBX_N_CLASSES], end_s[QBX_N_CLASSES];
+        hipEvent_t b0, b1;
+        bool pending;
+    };
+    std::vector<TimingSet> tsets;
+    int tnext = 0;
+    std::vector<double> tsum;      // summed per-class milliseconds of the drained sets
+    double tsum_total = 0.0;
+    int tcount = 0;
     double mixed_cut = 1e-5;
     double density_thr = 0.0;      // > 0: density-screened J/K
     bool xcd_local = false;        // per-XCD J/K copies (set by the probe at init)
@@ -122,6 +135,8 @@ struct qbx_ctx {
     int lds_max = 65536;           // LDS bytes per block on this device
     bool twins = true;             // twin-bra groups in the J/K build (QBX_TWINS=0: off)
     bool ket_buckets = true;       // kets bucketed by primitive-pair count (QBX_KET_BUCKETS=0: off)
+    bool jk_rows = true;           // LDS K-row and twin kernels in the J/K build (QBX_JK_ROWS=0: off, the
+                                   // one-wave-per-chunk kernels run everywhere)
     DevBuf<double> la_X, la_F, la_P, la_Q, la_T, la_aux;   // density purification
     std::mutex mu;
 };
@@ -200,6 +215,17 @@ struct qbx_basis {
     // work buffers
     DevBuf<double> d_D, d_J, d_K, d_G;
     DevBuf<double> d_pdmax;        // shell-pair max |D| for density screening
+    // end of the last Fock build (recorded on its stream after the
+    // symmetrization) and that stream: a build on another stream, and every
+    // rewrite of the task lists, waits for it before it touches the shared
+    // buffers (D, J, K, task lists, workspaces)
+    hipEvent_t ev_done = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool built = false;
+    ~qbx_basis()
+    {
+        if (ev_done) (void)hipEventDestroy(ev_done);
+    }
 };
 
 namespace {
@@ -1033,6 +1059,9 @@ void ensure_tasks(qbx_basis* b, double thr, int mode, double cut)
 {
     if (mode != QBX_PREC_MIXED) cut = 0.0;
     if (b->tasks_valid && b->task_thr == thr && b->task_mode == mode && b->task_cut == cut) return;
+    // the lists are rewritten (and may be reallocated) on the context stream:
+    // the last build, on whatever stream it ran, must be done reading them
+    if (b->built) QBX_HIP(hipEventSynchronize(b->ev_done));
     hipStream_t st = b->ctx->stream;
     for (int cls = 0; cls < QBX_N_CLASSES; ++cls) {
         TaskCounts tc;
@@ -1063,6 +1092,14 @@ void ensure_tasks(qbx_basis* b, double thr, int mode, double cut)
     b->task_thr = thr;
     b->task_mode = mode;
     b->task_cut = cut;
+}
+
+// Before a Fock build on stream st touches the basis's work buffers: a
+// previous build still running on another stream must finish first (stream
+// order covers a build on the same stream).
+void order_after_last_build(qbx_basis* b, hipStream_t st)
+{
+    if (b->built && b->last_stream != st) QBX_HIP(hipEventSynchronize(b->ev_done));
 }
 
 void check_ctx_basis(qbx_ctx* ctx, const qbx_basis* b)
@@ -1143,6 +1180,28 @@ void run_one_e(qbx_ctx* ctx, const qbx_basis* b, int kind, int nnuc, const doubl
     QBX_HIP(hipStreamSynchronize(st));
 }
 
+// Concurrent per-class timing: sum the per-class durations of a recorded
+// event set (each class's stretch on its auxiliary stream plus its stretch on
+// the main stream, where kernels that spill are launched) into the context's
+// accumulators.
+void drain_timing_set(qbx_ctx* ctx, qbx_ctx::TimingSet& T)
+{
+    if (!T.pending) return;
+    QBX_HIP(hipEventSynchronize(T.b1));
+    if (ctx->tsum.size() != (size_t)QBX_N_CLASSES) ctx->tsum.assign(QBX_N_CLASSES, 0.0);
+    for (int cls = 0; cls < QBX_N_CLASSES; ++cls) {
+        float l = 0.f, m = 0.f;
+        QBX_HIP(hipEventElapsedTime(&l, T.beg_l[cls], T.end_l[cls]));
+        QBX_HIP(hipEventElapsedTime(&m, T.beg_s[cls], T.end_s[cls]));
+        ctx->tsum[cls] += (double)l + (double)m;
+    }
+    float tot = 0.f;
+    QBX_HIP(hipEventElapsedTime(&tot, T.b0, T.b1));
+    ctx->tsum_total += tot;
+    ctx->tcount += 1;
+    T.pending = false;
+}
+
 void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* dG, double thr, int mode,
                hipStream_t st)
 {
@@ -1154,11 +1213,20 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
     b->d_K.ensure(NN * nset * copies);
     zero_async(b->d_J.p, NN * nset * copies, st);
     zero_async(b->d_K.p, NN * nset * copies, st);
+    // timing 1: every class on the main stream, one after another, bracketed
+    // by events; timing 2: the concurrent build as it runs untimed, each
+    // class bracketed by events on the stream it is launched on
     std::vector<hipEvent_t> ev;
-    if (ctx->timing) {
+    qbx_ctx::TimingSet* T = nullptr;
+    if (ctx->timing == 1) {
         ev.resize(2 * QBX_N_CLASSES + 2);
         for (auto& e : ev) QBX_HIP(hipEventCreate(&e));
         QBX_HIP(hipEventRecord(ev[2 * QBX_N_CLASSES], st));
+    } else if (ctx->timing == 2) {
+        T = &ctx->tsets[ctx->tnext];
+        drain_timing_set(ctx, *T);
+        ctx->tnext = (ctx->tnext + 1) % (int)ctx->tsets.size();
+        QBX_HIP(hipEventRecord(T->b0, st));
     }
     const double* pdmax = nullptr;
     const int ns = (int)b->shells.size();
@@ -1169,10 +1237,10 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
         QBX_HIP(hipGetLastError());
         pdmax = b->d_pdmax.p;
     }
-    // Without timing, the class kernels run concurrently on the auxiliary
-    // streams (staged classes first, then largest work first); they only meet
-    // in the J/K atomics.
-    const bool fork = !ctx->timing;
+    // Without serialized timing, the class kernels run concurrently on the
+    // auxiliary streams (staged classes first, then largest work first);
+    // they only meet in the J/K atomics.
+    const bool fork = ctx->timing != 1;
     if (fork) {
         QBX_HIP(hipEventRecord(ctx->ev_fork, st));
         for (int k = 0; k < kStreams; ++k) QBX_HIP(hipStreamWaitEvent(ctx->aux[k], ctx->ev_fork, 0));
@@ -1190,7 +1258,11 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
     for (int oi = 0; oi < QBX_N_CLASSES; ++oi) {
         const int cls = order[oi];
         hipStream_t ls = fork ? ctx->aux[oi % kStreams] : st;
-        if (ctx->timing) QBX_HIP(hipEventRecord(ev[2 * cls], st));
+        if (ctx->timing == 1) QBX_HIP(hipEventRecord(ev[2 * cls], st));
+        if (T) {
+            QBX_HIP(hipEventRecord(T->beg_l[cls], ls));
+            QBX_HIP(hipEventRecord(T->beg_s[cls], st));
+        }
         EriArgs a = base_args(b, cls);
         a.pdmax = pdmax;
         a.scratch_stream = (void*)st;
@@ -1203,7 +1275,7 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
         a.jk_stride = NN * nset;
         a.nset = nset;
         // twin groups where their LDS rows ([set][member][A and B][N]) fit a block
-        bool twin = b->use_twin[cls];
+        bool twin = b->use_twin[cls] && ctx->jk_rows;
         if (twin) {
             const size_t rows = (size_t)nset * b->w64[cls].twmax *
                                 (qbx_ncart(qbx_class_l[cls][0]) + qbx_ncart(qbx_class_l[cls][1])) * b->nbf *
@@ -1215,7 +1287,8 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
             a.prefix = t.prefix.p;
             a.koff = t.koff.p;
             a.cpre = t.cpre.p;
-            a.spre = t.spre.p;
+            // no K-row task lists: the one-wave-per-chunk kernels run
+            a.spre = ctx->jk_rows ? t.spre.p : nullptr;
             a.nsuper = t.nsuper;
             a.total = t.total;
             a.nwave = t.nwave;
@@ -1235,7 +1308,11 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
             set_tasks(b->t32[cls], nullptr);
             k_jk32[cls](a, ls);
         }
-        if (ctx->timing) QBX_HIP(hipEventRecord(ev[2 * cls + 1], st));
+        if (ctx->timing == 1) QBX_HIP(hipEventRecord(ev[2 * cls + 1], st));
+        if (T) {
+            QBX_HIP(hipEventRecord(T->end_l[cls], ls));
+            QBX_HIP(hipEventRecord(T->end_s[cls], st));
+        }
     }
     if (fork) {
         for (int k = 0; k < kStreams; ++k) {
@@ -1247,7 +1324,14 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
     hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
                        b->d_J.p, b->d_K.p, dG, b->nbf, nset, copies);
     QBX_HIP(hipGetLastError());
-    if (ctx->timing) {
+    QBX_HIP(hipEventRecord(b->ev_done, st));
+    b->last_stream = st;
+    b->built = true;
+    if (T) {
+        QBX_HIP(hipEventRecord(T->b1, st));
+        T->pending = true;
+    }
+    if (ctx->timing == 1) {
         QBX_HIP(hipEventRecord(ev[2 * QBX_N_CLASSES + 1], st));
         QBX_HIP(hipEventSynchronize(ev[2 * QBX_N_CLASSES + 1]));
         ctx->last_ms.assign(QBX_N_CLASSES, 0.f);
@@ -1331,6 +1415,8 @@ int qbx_init(int32_t device, qbx_ctx** out)
             c->twins = !(env && env[0] == '0');
             const char* benv = std::getenv("QBX_KET_BUCKETS");
             c->ket_buckets = !(benv && benv[0] == '0');
+            const char* renv = std::getenv("QBX_JK_ROWS");
+            c->jk_rows = !(renv && renv[0] == '0');
         }
         QBX_HIP(hipStreamSynchronize(c->stream));
         *out = c.release();
@@ -1351,6 +1437,16 @@ void qbx_finalize(qbx_ctx* ctx)
         (void)hipEventDestroy(ctx->ev_join[k]);
     }
     (void)hipEventDestroy(ctx->ev_fork);
+    for (auto& T : ctx->tsets) {
+        for (int c = 0; c < QBX_N_CLASSES; ++c) {
+            (void)hipEventDestroy(T.beg_l[c]);
+            (void)hipEventDestroy(T.end_l[c]);
+            (void)hipEventDestroy(T.beg_s[c]);
+            (void)hipEventDestroy(T.end_s[c]);
+        }
+        (void)hipEventDestroy(T.b0);
+        (void)hipEventDestroy(T.b1);
+    }
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1369,6 +1465,8 @@ int qbx_basis_create(qbx_ctx* ctx, int32_t n_prim, const double* prim_cen, const
         std::unique_ptr<qbx_basis> b(new qbx_basis());
         b->ctx = ctx;
         b->nbf = n_bf;
+        QBX_HIP(hipSetDevice(ctx->device));
+        QBX_HIP(hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming));
         // ---- group basis functions into shells ----
         struct Key {
             double c[3];
@@ -1551,6 +1649,7 @@ void qbx_basis_destroy(qbx_basis* b)
 {
     if (!b) return;
     (void)hipSetDevice(b->ctx->device);
+    if (b->built) (void)hipEventSynchronize(b->ev_done);
     (void)hipStreamSynchronize(b->ctx->stream);
     delete b;
 }
@@ -1660,6 +1759,7 @@ int qbx_jk_device(qbx_ctx* ctx, qbx_basis* b, int32_t n_set, const double* dDJ, 
         if (!(thr >= 0.0)) throw QbxError(QBX_EINVAL, "Schwarz threshold must be >= 0");
         QBX_HIP(hipSetDevice(ctx->device));
         hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+        order_after_last_build(b, st);
         const size_t NN = (size_t)b->nbf * b->nbf;
         // the kernels read DJ and DK from one contiguous buffer
         b->d_D.ensure(2 * NN * n_set);
@@ -1680,6 +1780,7 @@ int qbx_jk(qbx_ctx* ctx, qbx_basis* b, int32_t n_set, const double* DJ, const do
         if (!(thr >= 0.0)) throw QbxError(QBX_EINVAL, "Schwarz threshold must be >= 0");
         QBX_HIP(hipSetDevice(ctx->device));
         hipStream_t st = ctx->stream;
+        order_after_last_build(b, st);
         const size_t NN = (size_t)b->nbf * b->nbf;
         b->d_D.ensure(2 * NN * n_set);
         b->d_G.ensure(NN * n_set);
@@ -1904,11 +2005,34 @@ int qbx_jk_stats(qbx_ctx* ctx, qbx_basis* b, double thr, double cut, int64_t* sh
     });
 }
 
-int qbx_set_timing(qbx_ctx* ctx, int32_t enable)
+int qbx_set_timing(qbx_ctx* ctx, int32_t mode)
 {
     return guarded([&] {
         if (!ctx) throw QbxError(QBX_EINVAL, "null context");
-        ctx->timing = enable != 0;
+        if (mode < 0 || mode > 2) throw QbxError(QBX_EINVAL, "timing mode must be 0, 1 or 2");
+        QBX_HIP(hipSetDevice(ctx->device));
+        if (mode == 2 && ctx->tsets.empty()) {
+            ctx->tsets.resize(32);
+            for (auto& T : ctx->tsets) {
+                for (int c = 0; c < QBX_N_CLASSES; ++c) {
+                    QBX_HIP(hipEventCreate(&T.beg_l[c]));
+                    QBX_HIP(hipEventCreate(&T.end_l[c]));
+                    QBX_HIP(hipEventCreate(&T.beg_s[c]));
+                    QBX_HIP(hipEventCreate(&T.end_s[c]));
+                }
+                QBX_HIP(hipEventCreate(&T.b0));
+                QBX_HIP(hipEventCreate(&T.b1));
+                T.pending = false;
+            }
+        }
+        if (mode == 2) {
+            // start a fresh average
+            for (auto& T : ctx->tsets) T.pending = false;
+            ctx->tsum.assign(QBX_N_CLASSES, 0.0);
+            ctx->tsum_total = 0.0;
+            ctx->tcount = 0;
+        }
+        ctx->timing = mode;
     });
 }
 
@@ -1916,10 +2040,45 @@ int qbx_last_timing(qbx_ctx* ctx, float* ms, float* total)
 {
     return guarded([&] {
         if (!ctx) throw QbxError(QBX_EINVAL, "null context");
+        QBX_HIP(hipSetDevice(ctx->device));
+        if (ctx->timing == 2) {
+            // drain in recording order: the oldest pending set sits at tnext
+            const int n = (int)ctx->tsets.size();
+            for (int k = 0; k < n; ++k) drain_timing_set(ctx, ctx->tsets[(ctx->tnext + k) % n]);
+            if (ctx->tcount == 0) throw QbxError(QBX_EINVAL, "no timed qbx_jk call yet");
+            if (ms)
+                for (int c = 0; c < QBX_N_CLASSES; ++c) ms[c] = (float)(ctx->tsum[c] / ctx->tcount);
+            if (total) *total = (float)(ctx->tsum_total / ctx->tcount);
+            ctx->tsum.assign(QBX_N_CLASSES, 0.0);
+            ctx->tsum_total = 0.0;
+            ctx->tcount = 0;
+            return;
+        }
         if (ctx->last_ms.empty()) throw QbxError(QBX_EINVAL, "no timed qbx_jk call yet");
         if (ms)
             for (int c = 0; c < QBX_N_CLASSES; ++c) ms[c] = ctx->last_ms[c];
         if (total) *total = ctx->last_total;
+    });
+}
+
+int qbx_boys_device(qbx_ctx* ctx, int32_t order, int32_t n, const double* T, double* F)
+{
+    return guarded([&] {
+        if (!ctx) throw QbxError(QBX_EINVAL, "null context");
+        if (order < 0 || order > 8) throw QbxError(QBX_EINVAL, "Boys order must be in [0, 8]");
+        if (n <= 0) throw QbxError(QBX_EINVAL, "n must be positive");
+        if (!T || !F) throw QbxError(QBX_EINVAL, "null argument or output");
+        for (int i = 0; i < n; ++i)
+            if (!(T[i] >= 0.0 && T[i] <= 1e6)) throw QbxError(QBX_EINVAL, "Boys arguments must lie in [0, 1e6]");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        QBX_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        DevBuf<double> dT, dF;
+        dT.upload(std::vector<double>(T, T + n), st);
+        dF.ensure((size_t)n * (order + 1));
+        boys_probe(ctx->boysl.p + (size_t)order * QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE, order, dT.p, n, dF.p, st);
+        QBX_HIP(hipMemcpyAsync(F, dF.p, (size_t)n * (order + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+        QBX_HIP(hipStreamSynchronize(st));
     });
 }
 

@@ -108,4 +108,8 @@ template <int LA, int LB, int LC, int LD>
 void launch_eri_grad(const EriArgs& a, hipStream_t s);
 
 typedef void (*EriLaunch)(const EriArgs&, hipStream_t);
+
+// The ERI kernels' Boys function alone (qbx_boys_probe.hip): F_0..F_L at n
+// device arguments T from the order-L table rows, into F (n x (L+1)).
+void boys_probe(const double* rows, int L, const double* T, int n, double* F, hipStream_t s);
 }  // namespace qbx

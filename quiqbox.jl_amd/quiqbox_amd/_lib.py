@@ -45,6 +45,7 @@ SIGNATURES = {
     "qbx_last_timing": (_i32, [_vp, _pf, _pf]),
     "qbx_set_mixed_cut": (_i32, [_vp, _d]),
     "qbx_set_density_screening": (_i32, [_vp, _d]),
+    "qbx_boys_device": (_i32, [_vp, _i32, _i32, _pd, _pd]),
     "qbx_basis_shells": (_i32, [_vp, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd]),
     "qbx_rhf_gradient": (_i32, [_vp, _vp, _pd, _pd, _i32, _pd, _pd, _d, _pd, _pd, _pd]),
     "qbx_density_purify": (_i32, [_vp, _i32, _i32, _pd, _pd, _pd, _i32, _pi, _pd]),

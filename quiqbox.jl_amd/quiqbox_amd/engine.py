@@ -32,8 +32,10 @@ class Context:
             _lib.load().qbx_finalize(self.h)
             self.h = None
 
-    def set_timing(self, enable=True):
-        _lib.check(_lib.load().qbx_set_timing(self.h, int(bool(enable))))
+    def set_timing(self, mode=1):
+        """Per-class J/K kernel timing (qbx_set_timing): 1 serialized builds,
+        2 concurrent builds (averaged until the next last_timing), 0 off."""
+        _lib.check(_lib.load().qbx_set_timing(self.h, int(mode)))
 
     def last_timing(self):
         n = _lib.load().qbx_n_classes()
@@ -41,6 +43,15 @@ class Context:
         tot = ctypes.c_float()
         _lib.check(_lib.load().qbx_last_timing(self.h, ms, ctypes.byref(tot)))
         return np.array(ms[:], dtype=np.float64), float(tot.value)
+
+    def boys(self, order, T):
+        """F_0..F_order at the arguments T from the ERI kernels' Boys function
+        on the device (qbx_boys_device): array (len(T), order + 1)."""
+        T = np.ascontiguousarray(np.atleast_1d(T), dtype=np.float64)
+        F = np.zeros((len(T), order + 1))
+        with self.lock:
+            _lib.check(_lib.load().qbx_boys_device(self.h, int(order), len(T), _lib.dp(T), _lib.dp(F)))
+        return F
 
     def set_mixed_cut(self, cut):
         _lib.check(_lib.load().qbx_set_mixed_cut(self.h, float(cut)))

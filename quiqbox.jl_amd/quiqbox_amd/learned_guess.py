@@ -37,12 +37,37 @@ from .engine import GPUBasis
 from .hartree_fock import (HFconfig, OrbCoeffInitialConfig, RCHartreeFock, SCFconfig, getC,
                            runHartreeFock)
 
-NMAX = 15                     # functions per atom in the padded blocks (cc-pVDZ oxygen: 15 Cartesian)
-ELEMENTS = ("H", "O")
-N_FEAT = 2 * NMAX * NMAX + 2 * len(ELEMENTS) + 4
-N_IN = (N_FEAT + 7) // 8 * 8  # padded to the MFMA K step
-N_OUT = NMAX * NMAX
 WIDTH = 256
+
+
+class PairLayout:
+    """Elements the network knows and functions per atom in the padded blocks.
+    Features per atom pair: S and local core-Hamiltonian blocks (nmax^2 each),
+    element flags of both atoms, A == B, R, exp(-R), 1/(1+R)."""
+
+    def __init__(self, elements, nmax):
+        self.elements = tuple(elements)
+        self.nmax = int(nmax)
+        self.n_feat = 2 * self.nmax * self.nmax + 2 * len(self.elements) + 4
+        self.n_in = (self.n_feat + 7) // 8 * 8   # padded to the MFMA K step
+        self.n_out = self.nmax * self.nmax
+        self.dist_col = 2 * self.nmax * self.nmax + 2 * len(self.elements) + 1
+
+    def __repr__(self):
+        return "PairLayout(%s, nmax=%d)" % (self.elements, self.nmax)
+
+
+# cc-pVDZ water (oxygen: 15 Cartesian functions)
+WATER = PairLayout(("H", "O"), 15)
+# clusters of the random C/H/O generator with s and p shells on every atom (4 functions)
+CHO_SP = PairLayout(("H", "C", "O"), 4)
+
+# the default layout's sizes (water)
+NMAX = WATER.nmax
+ELEMENTS = WATER.elements
+N_FEAT = WATER.n_feat
+N_IN = WATER.n_in
+N_OUT = WATER.n_out
 
 
 def _torch():
@@ -52,7 +77,7 @@ def _torch():
     return torch
 
 
-def atom_blocks(nuc, gb):
+def atom_blocks(nuc, gb, layout=WATER):
     """Function indices of every atom (functions whose centre is the nucleus)."""
     cen = np.asarray(gb.flat.prim_cen)
     first = np.asarray(gb.flat.bf_prim)[np.asarray(gb.flat.bf_ptr)[:-1]]
@@ -62,18 +87,18 @@ def atom_blocks(nuc, gb):
     if np.any(np.sqrt(((fc - R[owner]) ** 2).sum(-1)) > 1e-8):
         raise ValueError("every basis function must sit on a nucleus")
     blocks = [np.nonzero(owner == a)[0] for a in range(len(R))]
-    if max(len(b) for b in blocks) > NMAX:
-        raise ValueError("atoms with more than %d functions" % NMAX)
+    if max(len(b) for b in blocks) > layout.nmax:
+        raise ValueError("atoms with more than %d functions" % layout.nmax)
     for s in nuc.symbols:
-        if s not in ELEMENTS:
-            raise ValueError("element %s outside the trained set %s" % (s, ELEMENTS))
+        if s not in layout.elements:
+            raise ValueError("element %s outside the trained set %s" % (s, layout.elements))
     return blocks
 
 
-def _padded_index(blocks, n):
-    """(n_atoms, NMAX) function indices of every atom, padded with n (a zero
+def _padded_index(blocks, n, nmax):
+    """(n_atoms, nmax) function indices of every atom, padded with n (a zero
     row/column appended to the matrices)."""
-    idx = np.full((len(blocks), NMAX), n, dtype=np.int64)
+    idx = np.full((len(blocks), nmax), n, dtype=np.int64)
     for a, b in enumerate(blocks):
         idx[a, :len(b)] = b
     return idx
@@ -86,59 +111,61 @@ def _ext(M):
     return E
 
 
-def pair_features(nuc, blocks, S, H):
+def pair_features(nuc, blocks, S, H, layout=WATER):
     """Raw features (n_atoms^2 x N_FEAT) of all ordered atom pairs.  H is
     (T, O): the kinetic matrix and the own-nucleus attraction O[i][j] (the
     nucleus at function i's centre alone, GPUBasis.nuclear_own), so that the
     pair (A, B) gets T + V(A) + V(B) = T + O + O^T on its block, and the pair
     (A, A) gets T + V(A)."""
     T, O = H
+    nm, els = layout.nmax, layout.elements
     na = len(blocks)
     n = S.shape[0]
-    idx = _padded_index(blocks, n)
+    idx = _padded_index(blocks, n, nm)
     I, J = idx[:, None, :, None], idx[None, :, None, :]
-    Sb = _ext(S)[I, J]                                  # (na, na, NMAX, NMAX)
+    Sb = _ext(S)[I, J]                                  # (na, na, nmax, nmax)
     OTb = _ext(O.T)[I, J]
     Hb = _ext(T)[I, J] + _ext(O)[I, J] + OTb
     diag = np.arange(na)
     Hb[diag, diag] -= OTb[diag, diag]                   # V(A) once on the atom's own block
-    X = np.zeros((na * na, N_FEAT))
-    X[:, :NMAX * NMAX] = Sb.reshape(na * na, -1)
-    X[:, NMAX * NMAX:2 * NMAX * NMAX] = Hb.reshape(na * na, -1)
-    o = 2 * NMAX * NMAX
-    el = np.array([ELEMENTS.index(sym) for sym in nuc.symbols])
+    X = np.zeros((na * na, layout.n_feat))
+    X[:, :nm * nm] = Sb.reshape(na * na, -1)
+    X[:, nm * nm:2 * nm * nm] = Hb.reshape(na * na, -1)
+    o = 2 * nm * nm
+    el = np.array([els.index(sym) for sym in nuc.symbols])
     X[np.arange(na * na), o + np.repeat(el, na)] = 1.0
-    X[np.arange(na * na), o + len(ELEMENTS) + np.tile(el, na)] = 1.0
+    X[np.arange(na * na), o + len(els) + np.tile(el, na)] = 1.0
     R = nuc.coord_array
     d = np.sqrt(((R[:, None, :] - R[None, :, :]) ** 2).sum(-1)).ravel()
-    X[:, o + 2 * len(ELEMENTS)] = np.eye(na).ravel()
-    X[:, o + 2 * len(ELEMENTS) + 1] = d
-    X[:, o + 2 * len(ELEMENTS) + 2] = np.exp(-d)
-    X[:, o + 2 * len(ELEMENTS) + 3] = 1.0 / (1.0 + d)
+    X[:, o + 2 * len(els)] = np.eye(na).ravel()
+    X[:, o + 2 * len(els) + 1] = d
+    X[:, o + 2 * len(els) + 2] = np.exp(-d)
+    X[:, o + 2 * len(els) + 3] = 1.0 / (1.0 + d)
     return X
 
 
-def pair_targets(blocks, D):
+def pair_targets(blocks, D, layout=WATER):
     na = len(blocks)
-    idx = _padded_index(blocks, D.shape[0])
-    return _ext(D)[idx[:, None, :, None], idx[None, :, None, :]].reshape(na * na, N_OUT)
+    idx = _padded_index(blocks, D.shape[0], layout.nmax)
+    return _ext(D)[idx[:, None, :, None], idx[None, :, None, :]].reshape(na * na, layout.n_out)
 
 
-def assemble_density(blocks, Y, n):
+def assemble_density(blocks, Y, n, layout=WATER):
     na = len(blocks)
-    idx = _padded_index(blocks, n)
+    nm = layout.nmax
+    idx = _padded_index(blocks, n, nm)
     D = np.zeros((n + 1, n + 1))
-    D[idx[:, None, :, None], idx[None, :, None, :]] = Y.reshape(na, na, NMAX, NMAX)
+    D[idx[:, None, :, None], idx[None, :, None, :]] = Y.reshape(na, na, nm, nm)
     D = D[:n, :n]
     return 0.5 * (D + D.T)
 
 
-def system_data(nuc, gb):
+def system_data(nuc, gb, layout=WATER):
     """S, (T, own-nucleus attraction), atom blocks; one-electron integrals from
     the GPU kernels."""
     S = gb.overlap()
     T = gb.kinetic()
-    blocks = atom_blocks(nuc, gb)
+    blocks = atom_blocks(nuc, gb, layout)
     O = gb.nuclear_own(nuc.charges, nuc.coord_array)
     return S, (T, O), blocks
 
@@ -169,20 +196,21 @@ def _p(t):
 class LearnedGuess:
     """The pair-feature MLP, its feature statistics and its GPU training loop."""
 
-    def __init__(self, seed=0):
+    def __init__(self, seed=0, layout=WATER):
         self.torch = _torch()
+        self.layout = layout
         self.rng = np.random.default_rng(seed)
-        self.sizes = [(N_IN, WIDTH), (WIDTH, WIDTH), (WIDTH, N_OUT)]
+        self.sizes = [(layout.n_in, WIDTH), (WIDTH, WIDTH), (WIDTH, layout.n_out)]
         self.dev = _Dev(self.torch, self.rng, self.sizes)
-        self.mu = np.zeros(N_FEAT)
-        self.sd = np.ones(N_FEAT)
+        self.mu = np.zeros(layout.n_feat)
+        self.sd = np.ones(layout.n_feat)
         self.step = 0
         self.r_max = np.inf      # largest atom-pair distance of the training set
 
     # ---- data ----
     def _inputs(self, X):
-        Z = np.zeros((X.shape[0], N_IN), dtype=np.float32)
-        Z[:, :N_FEAT] = (X - self.mu) / self.sd
+        Z = np.zeros((X.shape[0], self.layout.n_in), dtype=np.float32)
+        Z[:, :self.layout.n_feat] = (X - self.mu) / self.sd
         return Z
 
     def fit_stats(self, X):
@@ -220,12 +248,12 @@ class LearnedGuess:
         y, acts_b = self._forward(xb, n, keep=True)
         nreal = float(rw.sum().item())
         dY = torch.empty_like(y)
-        dYb = torch.empty((n, N_OUT), dtype=torch.bfloat16, device="cuda")
+        dYb = torch.empty((n, self.layout.n_out), dtype=torch.bfloat16, device="cuda")
         loss = torch.zeros(1, dtype=torch.float32, device="cuda")
         cmask = getattr(self, "_cmask", None)
         if cmask is None:
-            cmask = self._cmask = torch.ones(N_OUT, dtype=torch.float32, device="cuda")
-        _lib.check(L.qbx_mlp_mse(n, N_OUT, _p(y), _p(yt), _p(rw), _p(cmask), 1.0 / max(nreal, 1.0), _p(dY), _p(dYb),
+            cmask = self._cmask = torch.ones(self.layout.n_out, dtype=torch.float32, device="cuda")
+        _lib.check(L.qbx_mlp_mse(n, self.layout.n_out, _p(y), _p(yt), _p(rw), _p(cmask), 1.0 / max(nreal, 1.0), _p(dY), _p(dYb),
                                  _p(loss), None))
         grads = self.backward(acts_b, dY, dYb, n)
         self.step += 1
@@ -280,7 +308,7 @@ class LearnedGuess:
         X = np.concatenate([x for x, _ in data])
         Y = np.concatenate([y for _, y in data]).astype(np.float32)
         self.fit_stats(X)
-        self.r_max = float(X[:, 2 * NMAX * NMAX + 2 * len(ELEMENTS) + 1].max())
+        self.r_max = float(X[:, self.layout.dist_col].max())
         Z = self._inputs(X)
         n = len(Z)
         xb_all = torch.tensor(Z, device="cuda").to(torch.bfloat16)
@@ -301,22 +329,48 @@ class LearnedGuess:
                 log("epoch %d loss %.3e" % (ep, hist[-1]))
         return hist
 
+    # ---- persistence ----
+    def save(self, path, meta=None):
+        """Weights (FP32 masters), feature statistics and layout to an .npz
+        file (plain arrays; `meta` as a JSON string)."""
+        import json
+        arrs = {"W%d" % i: w.cpu().numpy() for i, w in enumerate(self.dev.W)}
+        arrs.update({"b%d" % i: b.cpu().numpy() for i, b in enumerate(self.dev.b)})
+        np.savez(path, mu=self.mu, sd=self.sd, r_max=np.array(self.r_max), nmax=np.array(self.layout.nmax),
+                 elements=np.array(self.layout.elements), meta=np.array(json.dumps(meta or {})), **arrs)
+
+    @classmethod
+    def load(cls, path):
+        """A network saved by `save` (read with allow_pickle=False)."""
+        import json
+        z = np.load(path, allow_pickle=False)
+        layout = PairLayout(tuple(str(e) for e in z["elements"]), int(z["nmax"]))
+        m = cls(seed=0, layout=layout)
+        for i in range(len(m.sizes)):
+            w = m.torch.tensor(z["W%d" % i], device="cuda")
+            m.dev.W[i].copy_(w)
+            m.dev.Wb[i].copy_(w.to(m.torch.bfloat16))
+            m.dev.b[i].copy_(m.torch.tensor(z["b%d" % i], device="cuda"))
+        m.mu, m.sd, m.r_max = z["mu"], z["sd"], float(z["r_max"])
+        m.meta = json.loads(str(z["meta"]))
+        return m
+
     # ---- guess ----
     def predict_density(self, nuc, gb, S=None, H=None):
         if S is None:
-            S, H, blocks = system_data(nuc, gb)
+            S, H, blocks = system_data(nuc, gb, self.layout)
         else:
-            blocks = atom_blocks(nuc, gb)
-        X = pair_features(nuc, blocks, S, H)
+            blocks = atom_blocks(nuc, gb, self.layout)
+        X = pair_features(nuc, blocks, S, H, self.layout)
         Y = self.predict(X)
-        Y[X[:, 2 * NMAX * NMAX + 2 * len(ELEMENTS) + 1] > self.r_max] = 0.0
-        return assemble_density(blocks, Y, gb.n)
+        Y[X[:, self.layout.dist_col] > self.r_max] = 0.0
+        return assemble_density(blocks, Y, gb.n, self.layout)
 
     def initial_coefficients(self, nuc, gb, S=None, H=None, schwarz=1e-12):
         """Coefficients from F(D_pred) = Hcore + J[D_pred] - K[D_pred]/2 (D_pred
         the predicted total density), diagonalized in the orthonormal basis."""
         if S is None:
-            S, H, _ = system_data(nuc, gb)
+            S, H, _ = system_data(nuc, gb, self.layout)
         Dt = self.predict_density(nuc, gb, S, H)
         Hcore = gb.core_hamiltonian(nuc.charges, nuc.coord_array)
         G = gb.jk(Dt, 0.5 * Dt, schwarz)
@@ -329,7 +383,7 @@ def guess_config(model, nuc, gb, strategy=None, maxStep=200, schwarz=1e-12, info
     or with the core-Hamiltonian guess when the predicted density is not usable.
     `info` (a dict), if given, receives the predicted electron count and the choice."""
     strategy = strategy or SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-8))
-    S, H, _ = system_data(nuc, gb)
+    S, H, _ = system_data(nuc, gb, model.layout)
     C, Dt = model.initial_coefficients(nuc, gb, S, H, schwarz=schwarz)
     n_e = float(np.sum(nuc.charges))
     n_pred = float(np.sum(Dt * S))
@@ -341,20 +395,45 @@ def guess_config(model, nuc, gb, strategy=None, maxStep=200, schwarz=1e-12, info
     return HFconfig(initial=OrbCoeffInitialConfig(RCHartreeFock(), (C,)), strategy=strategy, maxStep=maxStep)
 
 
-def training_set(systems, schwarz=1e-12, strategy=None, log=None):
+def training_set(systems, schwarz=1e-12, strategy=None, log=None, layout=WATER, maxStep=150, stats=None):
     """Converged GPU RHF densities of `systems` [(nuc, basis orbitals)] as
-    (features, targets) per system; systems that do not converge are skipped."""
+    (features, targets) per system; systems that do not converge are skipped.
+    `stats` (a dict), if given, receives the counts and the SCF time."""
     out = []
     t0 = time.perf_counter()
+    builds = 0
     for k, (nuc, bs) in enumerate(systems):
         gb = GPUBasis(bs)
-        S, H, blocks = system_data(nuc, gb)
-        cfg = HFconfig(initial="CoreH", strategy=strategy or SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-9)), maxStep=150)
+        S, H, blocks = system_data(nuc, gb, layout)
+        cfg = HFconfig(initial="CoreH", strategy=strategy or SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-9)),
+                       maxStep=maxStep)
         res = runHartreeFock(nuc, gb, cfg, printInfo=False, schwarzThreshold=schwarz)
+        builds += res.timing.get("fock_builds", 0)
         if not res.converged:
             continue
         Dt = 2.0 * res.density[0]
-        out.append((pair_features(nuc, blocks, S, H), pair_targets(blocks, Dt)))
+        out.append((pair_features(nuc, blocks, S, H, layout), pair_targets(blocks, Dt, layout)))
+    t = time.perf_counter() - t0
     if log:
-        log("training set: %d of %d systems converged, %.1f s" % (len(out), len(systems), time.perf_counter() - t0))
+        log("training set: %d of %d systems converged, %.1f s" % (len(out), len(systems), t))
+    if stats is not None:
+        stats.update(systems=len(systems), converged=len(out), scf_s=t, fock_builds=builds,
+                     pairs=int(sum(len(x) for x, _ in out)))
     return out
+
+
+def cho_cluster(n_atoms, seed):
+    """A closed-shell C/H/O cluster of the random s/p generator
+    (systems.random_cluster: per atom one 3-primitive s and one p shell with
+    random exponents and coefficients) at the 64-atom cluster's density: a
+    cube of 20 (n/64)^(1/3) bohr, minimum separation 1.8 bohr."""
+    from .systems import random_cluster
+    return random_cluster(n_atoms, seed, box=20.0 * (n_atoms / 64.0) ** (1.0 / 3.0))
+
+
+def cho_training_systems(count, seed0=100000, sizes=(8, 48)):
+    """`count` clusters of cho_cluster with atom counts drawn uniformly from
+    [sizes[0], sizes[1]] (seed seed0 + k for system k)."""
+    rng = np.random.default_rng(seed0)
+    ns = rng.integers(sizes[0], sizes[1] + 1, size=count)
+    return [cho_cluster(int(n), seed0 + k) for k, n in enumerate(ns)]

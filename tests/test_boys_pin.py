@@ -1,0 +1,145 @@
+"""The device Boys function, pinned through two-centre ERIs whose value is a
+fixed combination of F_0(T) .. F_L(T) at one argument T (tests/boys_pin.py).
+
+CPU: the exact evaluation itself against the committed 50-digit Boys table
+and against the oracle's primitive ERIs (so the reference formula is pinned
+independently of the device).
+GPU: every ERI class (total angular momentum L = 0..8, hence every order's LDS
+Boys table and its asymptotic cutoff T_L) through qbx_eri_dense, for T over
+[0, 150] including T_L -+ 0.05 for every L and points on and between the 0.25
+table grid, against the exact values to 1e-14 of the magnitude of the terms
+they sum.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import quiqbox_amd as qb
+from boys_pin import BOYS_TL, boys_mp, eri_two_centre
+from helpers import boys_table
+
+ALPHA, GAMMA = 0.8, 1.3
+RHO = (2 * ALPHA) * (2 * GAMMA) / (2 * ALPHA + 2 * GAMMA)
+
+
+def _components():
+    """(bra functions on A, ket functions on C) component pairs of every class:
+    shells (la, lb | lc, ld) with la >= lb, lc >= ld, (la, lb) >= (lc, ld);
+    components all along x, and z on both sides (y and z powers even in sum)."""
+    out = []
+    for la in range(3):
+        for lb in range(la + 1):
+            for lc in range(3):
+                for ld in range(lc + 1):
+                    if (la, lb) < (lc, ld):
+                        continue
+                    out.append(((la, 0, 0), (lb, 0, 0), (lc, 0, 0), (ld, 0, 0)))
+                    # z components where the total z power is even
+                    a = (la - 1, 0, 1) if la >= 1 else (0, 0, 0)
+                    c = (lc - 1, 0, 1) if lc >= 1 else (0, 0, 0)
+                    if (a[2] + c[2]) % 2 == 0 and (a, c) != ((la, 0, 0), (lc, 0, 0)):
+                        out.append((a, (lb, 0, 0), c, (ld, 0, 0)))
+    return out
+
+
+def _exact(comp, x, magnitude=False):
+    a, b, c, d = comp
+    bra = tuple(a[i] + b[i] for i in range(3))
+    ket = tuple(c[i] + d[i] for i in range(3))
+    return eri_two_centre(bra, ket, ALPHA, GAMMA, (0.0, 0.0, 0.0), (x, 0.0, 0.0), magnitude)
+
+
+def _t_values():
+    ts = [0.0, 1e-9, 0.03, 0.125, 0.5, 0.61, 1.0, 2.875, 7.25, 12.0, 20.1, 30.0]
+    for tl in BOYS_TL:
+        ts += [tl - 0.05, tl + 0.05]
+    ts += [75.0, 100.3, 150.0]
+    return ts
+
+
+def test_exact_reference_against_golden_boys_table():
+    rows = boys_table()["rows"]
+    worst = 0.0
+    for T, F in rows[::37]:
+        for m in (0, 3, 8, 16):
+            ref = F[m]
+            v = float(boys_mp(m, T))
+            worst = max(worst, abs(v - ref) / abs(ref))
+    assert worst < 2e-16
+
+
+def test_exact_reference_against_oracle_primitive_eris():
+    for comp in _components()[::3]:
+        for T in (0.0, 0.7, 13.0, 41.0, 120.0):
+            x = (T / RHO) ** 0.5
+            a, b, c, d = comp
+            A, C = (0.0, 0.0, 0.0), (x, 0.0, 0.0)
+            v = oracle.prim_eri((A, ALPHA, a), (A, ALPHA, b), (C, GAMMA, c), (C, GAMMA, d))
+            ref = _exact(comp, x)
+            assert v == pytest.approx(ref, rel=1e-12, abs=1e-300)
+
+
+@pytest.mark.gpu
+def test_device_boys_function_against_50_digit_values():
+    """The ERI kernels' Boys routine itself (qbx_boys_device: LDS rows of order
+    L, Taylor expansion, downward recurrence, asymptotic branch above T_L) for
+    every order L = 0..8 at every point of the committed 50-digit table (T in
+    [0, 150]) and at T_L -+ 0.05, F_0..F_L to 1e-14 relative."""
+    ctx = qb.default_context()
+    rows = boys_table()["rows"]
+    worst = 0.0
+    for L in range(9):
+        Ts = [r[0] for r in rows]
+        ref = {r[0]: r[1] for r in rows}
+        F = ctx.boys(L, Ts)
+        errL = 0.0
+        for i, T in enumerate(Ts):
+            for m in range(L + 1):
+                e = abs(F[i, m] - ref[T][m]) / ref[T][m]
+                errL = max(errL, e)
+        extra = [BOYS_TL[L] - 0.05, BOYS_TL[L] + 0.05, BOYS_TL[L] - 0.125, BOYS_TL[L] + 0.125]
+        Fx = ctx.boys(L, extra)
+        for i, T in enumerate(extra):
+            for m in range(L + 1):
+                r = float(boys_mp(m, T))
+                errL = max(errL, abs(Fx[i, m] - r) / r)
+        print("[gpu-test] Boys order %d: worst relative error %.3e over %d points" % (L, errL, len(Ts) + 4),
+              flush=True)
+        worst = max(worst, errL)
+    assert worst < 1e-14
+
+
+@pytest.mark.gpu
+def test_device_boys_every_order_against_exact():
+    comps = _components()
+    worst = 0.0
+    worst_at = None
+    for T in _t_values():
+        x = (T / RHO) ** 0.5
+        bs = []
+        for cen, xp in (((0.0, 0.0, 0.0), ALPHA), ((x, 0.0, 0.0), GAMMA)):
+            for L in range(3):
+                for ijk in qb.subshellXYZs(L):
+                    bs.append(qb.genGaussTypeOrb(cen, [xp], [1.0], ijk))
+        gb = qb.GPUBasis(bs)
+        E = gb.eri_dense()
+        ang = [tuple(int(v) for v in ijk) for L in range(3) for ijk in qb.subshellXYZs(L)]
+        idx = {a: k for k, a in enumerate(ang)}
+        off = len(ang)   # the ket functions (exponent GAMMA) follow the bra's
+        worst_T = 0.0
+        for comp in comps:
+            a, b, c, d = comp
+            v = E[idx[a], idx[b], off + idx[c], off + idx[d]]
+            ref, mag = _exact(comp, x, magnitude=True)
+            # relative to the magnitude of the terms the value sums: the
+            # recurrences' own cancellation (up to ~10x at large T) is not
+            # the Boys function's error
+            rel = abs(v - ref) / mag
+            worst_T = max(worst_T, rel)
+            if rel > worst:
+                worst, worst_at = rel, (T, comp, v, ref, mag)
+        print("[gpu-test] T = %.4g: %d components, worst error %.2e of the term magnitude" % (
+            T, len(comps), worst_T), flush=True)
+    print("[gpu-test] worst error %.3e of the term magnitude at T=%s %s (device %.17e exact %.17e, magnitude "
+          "%.3e)" % (worst, worst_at[0], worst_at[1], worst_at[2], worst_at[3], worst_at[4]), flush=True)
+    assert worst < 1e-14

@@ -72,6 +72,77 @@ def test_backward_matches_numpy_restatement():
         np.testing.assert_allclose(gb.cpu().numpy(), rb, atol=2e-3 * np.abs(rb).max() + 1e-6, rtol=0)
 
 
+def _loss(model, xb, yt, n, scale):
+    import ctypes
+    from quiqbox_amd import _lib
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    y = model._forward(xb, n)
+    no = model.layout.n_out
+    dY = torch.empty_like(y)
+    dYb = torch.empty((n, no), dtype=torch.bfloat16, device="cuda")
+    loss = torch.zeros(1, device="cuda")
+    rw = torch.ones(n, dtype=torch.float32, device="cuda")
+    cm = torch.ones(no, device="cuda")
+    _lib.check(_lib.load().qbx_mlp_mse(n, no, p(y), p(yt), p(rw), p(cm), scale, p(dY), p(dYb), p(loss), None))
+    torch.cuda.synchronize()
+    return float(loss.item()), y, dY, dYb
+
+
+def test_backward_against_central_differences():
+    """Each layer's weight and bias gradients from the GPU backward pass against
+    central differences of the GPU loss: L(W + dW) - L(W - dW) = g . (W+ - W-)
+    along random directions, W+- the BF16 weights the forward pass actually
+    uses.  The last layer's loss is quadratic in its weights and biases, so the
+    difference is exact up to FP32 rounding; earlier layers pass through the
+    BF16 rounding of the hidden activations, which leaves percent-level noise."""
+    model = lg.LearnedGuess(seed=5, layout=lg.CHO_SP)
+    rng = np.random.default_rng(6)
+    n = 64
+    lay = model.layout
+    X = rng.normal(size=(n, lay.n_in)).astype(np.float32)
+    T = rng.normal(size=(n, lay.n_out)).astype(np.float32)
+    xb = torch.tensor(X, device="cuda").to(torch.bfloat16)
+    yt = torch.tensor(T, device="cuda")
+    scale = 1.0 / n
+    _, y, dY, dYb = _loss(model, xb, yt, n, scale)
+    _, acts = model._forward(xb, n, keep=True)
+    grads = model.backward(acts, dY, dYb, n)
+    torch.cuda.synchronize()
+    d = model.dev
+    for li in range(3):
+        gW = grads[li][0].cpu().numpy().astype(np.float64)
+        gb = grads[li][1].cpu().numpy().astype(np.float64)
+        W0 = d.W[li].clone()
+        Wb0 = d.Wb[li].clone()
+        tol = 2e-3 if li == 2 else 5e-2
+        for trial in range(3):
+            V = torch.tensor(rng.choice([-1.0, 1.0], size=tuple(W0.shape)).astype(np.float32), device="cuda")
+            h = 0.02 * float(W0.abs().mean())
+            Wp = (W0 + h * V).to(torch.bfloat16)
+            Wm = (W0 - h * V).to(torch.bfloat16)
+            d.Wb[li].copy_(Wp)
+            Lp = _loss(model, xb, yt, n, scale)[0]
+            d.Wb[li].copy_(Wm)
+            Lm = _loss(model, xb, yt, n, scale)[0]
+            d.Wb[li].copy_(Wb0)
+            dW = (Wp.float() - Wm.float()).cpu().numpy().astype(np.float64)
+            pred = float(np.sum(gW * dW))
+            _say("layer %d weights, direction %d: L+ - L- %.6e, g . dW %.6e" % (li + 1, trial, Lp - Lm, pred))
+            assert abs((Lp - Lm) - pred) <= tol * abs(pred) + 1e-6
+        b0 = d.b[li].clone()
+        v = torch.tensor(rng.choice([-1.0, 1.0], size=tuple(b0.shape)).astype(np.float32), device="cuda")
+        hb = 0.05 * max(float(b0.abs().mean()), 0.1)
+        d.b[li].copy_(b0 + hb * v)
+        Lp = _loss(model, xb, yt, n, scale)[0]
+        d.b[li].copy_(b0 - hb * v)
+        Lm = _loss(model, xb, yt, n, scale)[0]
+        d.b[li].copy_(b0)
+        db = ((b0 + hb * v) - (b0 - hb * v)).cpu().numpy().astype(np.float64)
+        pred = float(np.sum(gb * db))
+        _say("layer %d biases: L+ - L- %.6e, g . db %.6e" % (li + 1, Lp - Lm, pred))
+        assert abs((Lp - Lm) - pred) <= tol * abs(pred) + 1e-6
+
+
 def test_trained_network_beats_the_core_guess_on_an_unseen_cluster():
     systems = [water_box(n, 100 + n * 7 + k, basis="cc-pVDZ") for n in (2, 3, 4) for k in range(4)]
     data = lg.training_set(systems, log=_say)

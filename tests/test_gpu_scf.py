@@ -37,8 +37,30 @@ def test_hho_rhf_uhf_against_reference_and_oracle():
     ref = qb.runHartreeFock(nuc, OracleEngine(bs), qb.HFconfig(initial="CoreH", strategy=scf), printInfo=False)
     assert abs(res.energy[0] - ref.energy[0]) < 1e-10
     _say("HHO UHF")
-    ures = qb.runHartreeFock(nuc, bs, qb.HFconfig(qb.UOHartreeFock(), initial="GWH", strategy=scf), printInfo=False)
+    ucfg = qb.HFconfig(qb.UOHartreeFock(), initial="GWH", strategy=scf)
+    ures = qb.runHartreeFock(nuc, bs, ucfg, printInfo=False)
     assert ures.energy[0] == pytest.approx(L["E_uhf"], abs=100 * L["tol_E"])
+    uref = qb.runHartreeFock(nuc, OracleEngine(bs), ucfg, printInfo=False)
+    _say("  UHF E_gpu %.12f E_oracle %.12f" % (ures.energy[0], uref.energy[0]))
+    assert abs(ures.energy[0] - uref.energy[0]) < 1e-10
+
+
+@pytest.mark.parametrize("R", [2.5, 4.1, 6.1])
+def test_stretched_h2_uhf_matches_oracle(R):
+    """UHF energies of the GPU engine against the oracle-driven UHF (dense
+    tensor getGcore) at 1e-10 Eh where UHF breaks the spin symmetry: H2 in
+    3-21G at 2.5, 4.1 and 6.1 bohr from the core-Hamiltonian guess."""
+    coords = [(0.0, 0.0, 0.0), (R, 0.0, 0.0)]
+    nuc = qb.NuclearCluster(["H", "H"], coords)
+    bs = molecule_basis(["H", "H"], coords, "3-21G")
+    cfg = qb.HFconfig(qb.UOHartreeFock(), initial="CoreH", maxStep=300)
+    _say("H2 3-21G R=%.1f UHF GPU vs oracle" % R)
+    res = qb.runHartreeFock(nuc, bs, cfg, printInfo=False)
+    ref = qb.runHartreeFock(nuc, OracleEngine(bs), cfg, printInfo=False)
+    rhf = qb.runHartreeFock(nuc, bs, qb.HFconfig(initial="CoreH", maxStep=300), printInfo=False)
+    _say("  E_gpu %.12f E_oracle %.12f (RHF %.12f)" % (res.energy[0], ref.energy[0], rhf.energy[0]))
+    assert res.converged and ref.converged
+    assert abs(res.energy[0] - ref.energy[0]) < 1e-10
 
 
 def test_h2o2_631g_default_config():
@@ -70,12 +92,13 @@ def test_config2_small_cluster_matches_oracle():
     """16-atom cluster of the config-2 generator (64 functions): GPU direct SCF
     vs oracle dense-tensor SCF from the same core-Hamiltonian guess."""
     nuc, bs = random_cluster(16, 3, box=10.0)
-    cfg = qb.HFconfig(initial="CoreH", strategy=qb.SCFconfig(("DIIS",), (1e-10,)), maxStep=100)
+    cfg = qb.HFconfig(initial="CoreH", strategy=qb.SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-10)), maxStep=200)
     _say("16-atom cluster GPU SCF n_bf=%d" % len(bs))
     res = qb.runHartreeFock(nuc, bs, cfg, printInfo=False, schwarzThreshold=0.0)
     ref = qb.runHartreeFock(nuc, OracleEngine(bs), cfg, printInfo=False)
     _say("  E_gpu %.12f E_oracle %.12f conv %s %s" % (res.energy[0], ref.energy[0], res.converged, ref.converged))
-    assert abs(res.energy[0] - ref.energy[0]) < 1e-9
+    assert res.converged and ref.converged
+    assert abs(res.energy[0] - ref.energy[0]) < 1e-10
 
 
 def test_incremental_and_mixed_scf_match_plain_fp64():

@@ -239,3 +239,48 @@ def test_reference_orientation_loses_digits_stable_one_does_not():
     v = ob.eri(*q)
     assert abs(v - ref) < 1e-13
     assert abs(v_ref - ref) > 1e-10
+
+
+def test_multipole_off_centre_closed_forms():
+    """Moments about a point M away from both centres, at one of them (the
+    swap branch) and of a primitive with itself, against closed forms of the
+    Gaussian product exp(-a r_A^2) exp(-b r_B^2) = K exp(-p r_P^2):
+    <s|(x-M)^n|s> = S (u^n + C(n,2) u^{n-2}/(2p) + 3 [n=4]/(4 p^2)), u = P_x - M_x,
+    <x_A|(x-M)|s> = S ((P_x - A_x) u + 1/(2p))."""
+    import math
+    a, b = 0.9, 0.35
+    A = (0.3, -0.2, 0.5)
+    B = (-0.4, 0.6, 1.1)
+    p = a + b
+    P = [(a * A[i] + b * B[i]) / p for i in range(3)]
+    K = math.exp(-a * b / p * sum((A[i] - B[i]) ** 2 for i in range(3)))
+    S = K * (math.pi / p) ** 1.5
+
+    def s_moment(u, n):
+        return {1: u, 2: u * u + 1 / (2 * p), 3: u ** 3 + 3 * u / (2 * p),
+                4: u ** 4 + 6 * u * u / (2 * p) + 3 / (4 * p * p)}[n]
+
+    s0 = (0, 0, 0)
+    for M in ((0.7, 0.1, -0.3), A, B):
+        for t in range(3):
+            for n in (1, 2, 3, 4):
+                deg = [0, 0, 0]
+                deg[t] = n
+                u = P[t] - M[t]
+                ref = S * s_moment(u, n)
+                v = oracle.prim_multipole((A, a, s0), (B, b, s0), M, deg)
+                assert v == pytest.approx(ref, rel=1e-13, abs=1e-15), (M, t, n)
+                v2 = oracle.prim_multipole((B, b, s0), (A, a, s0), M, deg)
+                assert v2 == pytest.approx(ref, rel=1e-13, abs=1e-15), (M, t, n)
+        # p function on A: <x_A|(x - M)|s_B>
+        u = P[0] - M[0]
+        ref = S * ((P[0] - A[0]) * u + 1 / (2 * p))
+        v = oracle.prim_multipole((A, a, (1, 0, 0)), (B, b, s0), M, (1, 0, 0))
+        assert v == pytest.approx(ref, rel=1e-13, abs=1e-15)
+        v2 = oracle.prim_multipole((B, b, s0), (A, a, (1, 0, 0)), M, (1, 0, 0))
+        assert v2 == pytest.approx(ref, rel=1e-13, abs=1e-15)
+    # a primitive with itself about an outside point: P = A, p = 2a
+    M = (0.7, 0.1, -0.3)
+    Sa = (math.pi / (2 * a)) ** 1.5
+    v = oracle.prim_multipole((A, a, s0), (A, a, s0), M, (2, 0, 0))
+    assert v == pytest.approx(Sa * ((A[0] - M[0]) ** 2 + 1 / (4 * a)), rel=1e-13)
