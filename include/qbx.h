@@ -214,6 +214,22 @@ int  qbx_rhf_gradient(qbx_ctx* ctx, qbx_basis* basis, const double* Dt, const do
 int  qbx_density_purify(qbx_ctx* ctx, int32_t n, int32_t nocc, const double* X, const double* F, double* D,
                         int32_t max_iter, int32_t* iterations, double* idempotency);
 
+/* The same density without allocating: no counterpart in the reference, for
+   callers that cannot allocate device memory (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc).  X, F (inputs) and D
+   (output) are n x n row-major DEVICE arrays; the work matrices live in a
+   fixed device array of the library (3 n^2 + 2 n + 4 doubles for n up to
+   QBX_PURIFY_NOALLOC_MAX_N: 25 MB per GPU, held until the process ends), so
+   n above the maximum is QBX_EINVAL.  Enqueued on `stream` (NULL: the
+   context stream), which it synchronizes to read the traces; returns with D
+   written.  Every call on one GPU shares that array: calls that overlap on
+   one GPU race.  Nothing in it needs clearing between calls (every entry is
+   written before it is read). */
+#define QBX_PURIFY_NOALLOC_MAX_N 1024
+int  qbx_density_purify_noalloc(qbx_ctx* ctx, int32_t n, int32_t nocc, const double* dX, const double* dF,
+                                double* dD, int32_t max_iter, int32_t* iterations, double* idempotency,
+                                void* stream);
+
 /* C[M x N] = alpha op(A) op(B) + beta C in FP64 on the matrix cores
    (v_mfma_f64_16x16x4f64), row-major device matrices, op as for
    qbx_gemm_bf16; enqueued on `stream` (NULL: default stream), no sync.  The

@@ -98,6 +98,16 @@ constexpr double QBX_PRIM_PAIR_CUT = 1e-20;
 // ket range per bucket.  Estimated on the (ss|ss) kets of 24 H2O in cc-pVDZ:
 // lanes busy 36% -> 78% of the primitive loop, 3.5% more waves.
 constexpr int kKetBuckets = 5;
+// Fixed device work array of qbx_density_purify_noalloc: P, Q, T (n x n) and
+// 2n + 4 auxiliary values for n up to QBX_PURIFY_NOALLOC_MAX_N, 25 MB per
+// GPU for the life of the process.  Host code never uses the symbol itself as
+// an address: qbx_init takes each device's address with hipGetSymbolAddress.
+constexpr size_t kPurifyScratch = 3 * (size_t)QBX_PURIFY_NOALLOC_MAX_N * QBX_PURIFY_NOALLOC_MAX_N +
+                                  2 * (size_t)QBX_PURIFY_NOALLOC_MAX_N + 4;
+static __device__ __align__(256) double scratch[kPurifyScratch];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_addr[kMaxDevices] = {};
 
 inline int ket_bucket(int ppn) { return ppn <= 1 ? 0 : ppn <= 4 ? 1 : ppn <= 9 ? 2 : ppn <= 27 ? 3 : 4; }
 
@@ -138,6 +148,7 @@ struct qbx_ctx {
     bool jk_rows = true;           // LDS K-row and twin kernels in the J/K build (QBX_JK_ROWS=0: off, the
                                    // one-wave-per-chunk kernels run everywhere)
     DevBuf<double> la_X, la_F, la_P, la_Q, la_T, la_aux;   // density purification
+    double* scratch = nullptr;     // this device's address of the fixed work array
     std::mutex mu;
 };
 
@@ -1342,6 +1353,69 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
     }
 }
 
+// Work matrices of the purification: P, Q, T (n x n each) and aux (2n + 4).
+struct PurifyWork {
+    double *P, *Q, *T, *aux;
+};
+
+// D = C[:, :nocc] C[:, :nocc]^T of eigen(X' F X) by trace-correcting
+// purification (device X and F, n x n row-major) on stream st, using only the
+// work matrices w; returns the device pointer (one of w's matrices) holding D.
+// QBX_ENOCONV when the trace or the idempotency has not settled.
+const double* purify_density(int n, int nocc, const double* dX, const double* dF, const PurifyWork& w, int max_iter,
+                             int32_t* iterations, double* idempotency, hipStream_t st)
+{
+    double* dP = w.P;
+    double* dQ = w.Q;
+    double* dT = w.T;
+    // Fock matrix in the orthonormal basis: Fo = X' F X (into dQ)
+    gemm_f64(false, false, n, n, n, 1.0, dF, n, dX, n, 0.0, nullptr, 0, dT, n, st);
+    gemm_f64(true, false, n, n, n, 1.0, dX, n, dT, n, 0.0, nullptr, 0, dQ, n, st);
+    // spectral interval from Gershgorin discs
+    double lohi[2];
+    gershgorin(dQ, n, w.aux, st);
+    spectral_interval(w.aux, n, w.aux + 2 * (size_t)n, st);
+    QBX_HIP(hipMemcpyAsync(lohi, w.aux + 2 * (size_t)n, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    QBX_HIP(hipStreamSynchronize(st));
+    double lo = lohi[0], hi = lohi[1];
+    if (!(hi > lo)) hi = lo + 1.0;
+    sp2_start(dQ, n, lo, hi, dP, st);
+    // trace-correcting purification (SP2): P <- P^2 or 2P - P^2, whichever
+    // moves trace(P) toward nocc; quadratic convergence to the projector
+    // on the nocc lowest eigenvectors once the trace is right
+    double tf[2];
+    double* dtf = w.aux + 2 * (size_t)n + 2;
+    auto measure = [&] {
+        trace_and_frobenius(dP, n, dtf, st);
+        QBX_HIP(hipMemcpyAsync(tf, dtf, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        QBX_HIP(hipStreamSynchronize(st));
+    };
+    measure();
+    int it = 0, tail = -1;
+    const double N = (double)nocc;
+    for (; it < max_iter; ++it) {
+        const double tr = tf[0], fr = tf[1];
+        const double err = std::fabs(tr - fr);
+        if (tail < 0 && err < 1e-8 * std::max(1.0, N) && std::fabs(tr - N) < 1e-6) tail = 2;
+        if (tail == 0) break;
+        if (tail > 0) --tail;
+        if (std::fabs(fr - N) <= std::fabs(2.0 * tr - fr - N))
+            gemm_f64(false, false, n, n, n, 1.0, dP, n, dP, n, 0.0, nullptr, 0, dQ, n, st);
+        else
+            gemm_f64(false, false, n, n, n, -1.0, dP, n, dP, n, 2.0, dP, n, dQ, n, st);
+        std::swap(dP, dQ);
+        measure();
+    }
+    if (iterations) *iterations = it;
+    if (idempotency) *idempotency = std::fabs(tf[0] - tf[1]);
+    if (tail != 0 || !(std::fabs(tf[0] - N) < 1e-6))
+        throw QbxError(QBX_ENOCONV, "purification did not converge (no gap at the Fermi level, or max_iter too small)");
+    // D = X P X'
+    gemm_f64(false, false, n, n, n, 1.0, dX, n, dP, n, 0.0, nullptr, 0, dT, n, st);
+    gemm_f64(false, true, n, n, n, 1.0, dT, n, dX, n, 0.0, nullptr, 0, dQ, n, st);
+    return dQ;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1385,6 +1459,18 @@ int qbx_init(int32_t device, qbx_ctx** out)
         c->bexp.upload(et, c->stream);
         c->boysl.upload(rows, c->stream);
         QBX_HIP(hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+        // address of the fixed work array on this device (device current), once per device
+        {
+            if (device >= kMaxDevices) throw QbxError(QBX_EUNSUP, "device index above the library's table");
+            std::lock_guard<std::mutex> lk(g_scratch_mu);
+            if (!g_scratch_addr[device]) {
+                void* p = nullptr;
+                QBX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+                if (!p) throw QbxError(QBX_EHIP, "hipGetSymbolAddress gave no address for the work array");
+                g_scratch_addr[device] = (double*)p;
+            }
+            c->scratch = g_scratch_addr[device];
+        }
         // Per-XCD J/K accumulation needs the XCC_ID register to name each
         // block's XCD: check that it reads ids below QBX_MAX_XCD and that
         // several XCDs show up (a single-XCD device keeps one copy).
@@ -1809,63 +1895,39 @@ int qbx_density_purify(qbx_ctx* ctx, int32_t n, int32_t nocc, const double* X, c
         ctx->la_P.ensure(nn);
         ctx->la_Q.ensure(nn);
         ctx->la_T.ensure(nn);
-        ctx->la_aux.ensure(2 * (size_t)n + 2);
-        double* dX = ctx->la_X.p;
-        double* dF = ctx->la_F.p;
-        double* dP = ctx->la_P.p;
-        double* dQ = ctx->la_Q.p;
-        double* dT = ctx->la_T.p;
-        QBX_HIP(hipMemcpyAsync(dX, X, nn * sizeof(double), hipMemcpyHostToDevice, st));
-        QBX_HIP(hipMemcpyAsync(dF, F, nn * sizeof(double), hipMemcpyHostToDevice, st));
-        // Fock matrix in the orthonormal basis: Fo = X' F X (into dQ)
-        gemm_f64(false, false, n, n, n, 1.0, dF, n, dX, n, 0.0, nullptr, 0, dT, n, st);
-        gemm_f64(true, false, n, n, n, 1.0, dX, n, dT, n, 0.0, nullptr, 0, dQ, n, st);
-        // spectral interval from Gershgorin discs
-        std::vector<double> lohi(2 * (size_t)n);
-        gershgorin(dQ, n, ctx->la_aux.p, st);
-        QBX_HIP(hipMemcpyAsync(lohi.data(), ctx->la_aux.p, 2 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+        ctx->la_aux.ensure(2 * (size_t)n + 4);
+        QBX_HIP(hipMemcpyAsync(ctx->la_X.p, X, nn * sizeof(double), hipMemcpyHostToDevice, st));
+        QBX_HIP(hipMemcpyAsync(ctx->la_F.p, F, nn * sizeof(double), hipMemcpyHostToDevice, st));
+        const PurifyWork w{ctx->la_P.p, ctx->la_Q.p, ctx->la_T.p, ctx->la_aux.p};
+        const double* dD = purify_density(n, nocc, ctx->la_X.p, ctx->la_F.p, w, max_iter, iterations, idempotency, st);
+        QBX_HIP(hipMemcpyAsync(D, dD, nn * sizeof(double), hipMemcpyDeviceToHost, st));
         QBX_HIP(hipStreamSynchronize(st));
-        double lo = lohi[0], hi = lohi[1];
-        for (int i = 1; i < n; ++i) {
-            lo = std::min(lo, lohi[2 * i]);
-            hi = std::max(hi, lohi[2 * i + 1]);
-        }
-        if (!(hi > lo)) hi = lo + 1.0;
-        sp2_start(dQ, n, lo, hi, dP, st);
-        // trace-correcting purification (SP2): P <- P^2 or 2P - P^2, whichever
-        // moves trace(P) toward nocc; quadratic convergence to the projector
-        // on the nocc lowest eigenvectors once the trace is right
-        double tf[2];
-        double* dtf = ctx->la_aux.p + 2 * (size_t)n;
-        auto measure = [&] {
-            trace_and_frobenius(dP, n, dtf, st);
-            QBX_HIP(hipMemcpyAsync(tf, dtf, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-            QBX_HIP(hipStreamSynchronize(st));
-        };
-        measure();
-        int it = 0, tail = -1;
-        const double N = (double)nocc;
-        for (; it < max_iter; ++it) {
-            const double tr = tf[0], fr = tf[1];
-            const double err = std::fabs(tr - fr);
-            if (tail < 0 && err < 1e-8 * std::max(1.0, N) && std::fabs(tr - N) < 1e-6) tail = 2;
-            if (tail == 0) break;
-            if (tail > 0) --tail;
-            if (std::fabs(fr - N) <= std::fabs(2.0 * tr - fr - N))
-                gemm_f64(false, false, n, n, n, 1.0, dP, n, dP, n, 0.0, nullptr, 0, dQ, n, st);
-            else
-                gemm_f64(false, false, n, n, n, -1.0, dP, n, dP, n, 2.0, dP, n, dQ, n, st);
-            std::swap(dP, dQ);
-            measure();
-        }
-        if (iterations) *iterations = it;
-        if (idempotency) *idempotency = std::fabs(tf[0] - tf[1]);
-        if (tail != 0 || !(std::fabs(tf[0] - N) < 1e-6))
-            throw QbxError(QBX_ENOCONV, "purification did not converge (no gap at the Fermi level, or max_iter too small)");
-        // D = X P X'
-        gemm_f64(false, false, n, n, n, 1.0, dX, n, dP, n, 0.0, nullptr, 0, dT, n, st);
-        gemm_f64(false, true, n, n, n, 1.0, dT, n, dX, n, 0.0, nullptr, 0, dQ, n, st);
-        QBX_HIP(hipMemcpyAsync(D, dQ, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    });
+}
+
+int qbx_density_purify_noalloc(qbx_ctx* ctx, int32_t n, int32_t nocc, const double* dX, const double* dF, double* dD,
+                               int32_t max_iter, int32_t* iterations, double* idempotency, void* stream)
+{
+    return guarded([&] {
+        if (!ctx) throw QbxError(QBX_EINVAL, "null context");
+        if (n <= 0 || nocc < 0 || nocc > n) throw QbxError(QBX_EINVAL, "need 0 <= nocc <= n and n > 0");
+        if (n > QBX_PURIFY_NOALLOC_MAX_N)
+            throw QbxError(QBX_EINVAL, "qbx_density_purify_noalloc takes n <= " +
+                                           std::to_string(QBX_PURIFY_NOALLOC_MAX_N) + " (its fixed work array); got " +
+                                           std::to_string(n));
+        if (!dX || !dF || !dD) throw QbxError(QBX_EINVAL, "null matrix pointer");
+        if (max_iter <= 0) throw QbxError(QBX_EINVAL, "max_iter must be positive");
+        if (!ctx->scratch) throw QbxError(QBX_EINTERNAL, "the fixed work array has no device address");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        QBX_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+        // the work matrices in the library's fixed device array (its address on
+        // this device, taken at qbx_init); every entry is written before it is read
+        const size_t nn = (size_t)n * n;
+        double* base = ctx->scratch;
+        const PurifyWork w{base, base + nn, base + 2 * nn, base + 3 * nn};
+        const double* dOut = purify_density(n, nocc, dX, dF, w, max_iter, iterations, idempotency, st);
+        QBX_HIP(hipMemcpyAsync(dD, dOut, nn * sizeof(double), hipMemcpyDeviceToDevice, st));
         QBX_HIP(hipStreamSynchronize(st));
     });
 }

@@ -18,6 +18,9 @@ void pair_columns(const double* C, int nao, int nmo, double* out, hipStream_t s)
 // lohi[2i] = P_ii - sum_j!=i |P_ij|, lohi[2i+1] = P_ii + sum_j!=i |P_ij|
 void gershgorin(const double* P, int n, double* lohi, hipStream_t s);
 
+// out[0] = min of lohi[2i], out[1] = max of lohi[2i+1] over the n rows
+void spectral_interval(const double* lohi, int n, double* out, hipStream_t s);
+
 // P = (hi I - F) / (hi - lo)
 void sp2_start(const double* F, int n, double lo, double hi, double* P, hipStream_t s);
 

@@ -176,6 +176,37 @@ __global__ __launch_bounds__(1024) void trace_frob_kernel(const double* P, int n
     }
 }
 
+// out[0] = min_i lohi[2i], out[1] = max_i lohi[2i+1]: the spectral interval
+// from the rows' Gershgorin intervals (one block; min and max are exact in
+// any order)
+__global__ __launch_bounds__(1024) void interval_kernel(const double* lohi, int n, double* out)
+{
+    __shared__ double s_lo[16], s_hi[16];
+    double lo = lohi[0], hi = lohi[1];
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        lo = fmin(lo, lohi[2 * i]);
+        hi = fmax(hi, lohi[2 * i + 1]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, o));
+        hi = fmax(hi, __shfl_xor(hi, o));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_lo[w] = lo;
+        s_hi[w] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 16; ++k) {
+            lo = fmin(lo, s_lo[k]);
+            hi = fmax(hi, s_hi[k]);
+        }
+        out[0] = fmin(lo, s_lo[0]);
+        out[1] = fmax(hi, s_hi[0]);
+    }
+}
+
 }  // namespace
 
 void gemm_f64(bool trans_a, bool trans_b, int M, int N, int K, double alpha, const double* A, int lda,
@@ -221,6 +252,12 @@ void pair_columns(const double* C, int nao, int nmo, double* out, hipStream_t s)
 void gershgorin(const double* P, int n, double* lohi, hipStream_t s)
 {
     hipLaunchKernelGGL(gershgorin_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P, n, lohi);
+    QBX_HIP(hipGetLastError());
+}
+
+void spectral_interval(const double* lohi, int n, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(interval_kernel, dim3(1), dim3(1024), 0, s, lohi, n, out);
     QBX_HIP(hipGetLastError());
 }
 

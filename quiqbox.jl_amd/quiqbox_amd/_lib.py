@@ -49,6 +49,7 @@ SIGNATURES = {
     "qbx_basis_shells": (_i32, [_vp, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd]),
     "qbx_rhf_gradient": (_i32, [_vp, _vp, _pd, _pd, _i32, _pd, _pd, _d, _pd, _pd, _pd]),
     "qbx_density_purify": (_i32, [_vp, _i32, _i32, _pd, _pd, _pd, _i32, _pi, _pd]),
+    "qbx_density_purify_noalloc": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp, _i32, _pi, _pd, _vp]),
     "qbx_mo_transform": (_i32, [_vp, _i32, _i32, _i32, _pd, _pd, _pd]),
     "qbx_mo_coulomb_ab": (_i32, [_vp, _i32, _i32, _i32, _pd, _pd, _pd, _pd]),
     "qbx_gemm_f64": (_i32, [_i32, _i32, _i32, _i32, _i32, _d, _vp, _i32, _vp, _i32, _d, _vp, _i32, _vp]),
