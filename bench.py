@@ -410,6 +410,9 @@ def main():
                     help="orbital-basis change of an n^4 two-body tensor (0: skip it)")
     ap.add_argument("--serial", action="store_true",
                     help="run the class kernels of each build one after another on one stream (for profiling)")
+    ap.add_argument("--timed-only", action="store_true",
+                    help="skip the serialized per-class builds after the timed region (profiling runs: every "
+                         "J/K dispatch is then one of the concurrent builds)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -463,14 +466,16 @@ def main():
                                                                                  conc_total))
 
     # the same classes one at a time (serialized builds, outside the timed region)
-    gb.ctx.set_timing(1)
-    ser = np.zeros(len(classes))
-    reps = max(1, min(3, args.steps))
-    for _ in range(reps):
-        gb.jk_device(dDJ, dDK, dG, 1, thr)
-        ms, tot = gb.ctx.last_timing()
-        ser += ms
-    ser /= reps
+    ser = np.full(len(classes), np.nan)
+    if not args.timed_only:
+        gb.ctx.set_timing(1)
+        ser[:] = 0.0
+        reps = max(1, min(3, args.steps))
+        for _ in range(reps):
+            gb.jk_device(dDJ, dDK, dG, 1, thr)
+            ms, tot = gb.ctx.last_timing()
+            ser += ms
+        ser /= reps
     gb.ctx.set_timing(0)
     # dominant kernel: the largest share of the concurrent build
     dom = int(np.argmax(conc))
@@ -512,8 +517,9 @@ def main():
                      "kernel_ms": float(conc[dom]), "kernel_flops": float(cflops[dom]),
                      "timing": "average duration in the concurrent builds of the timed region (HIP events "
                                "on the kernel's stream)",
-                     "serialized_ms": float(ser[dom]), "serialized_tflops": achieved_ser,
-                     "serialized_frac": achieved_ser / PEAK_FP64_TFLOPS,
+                     "serialized_ms": None if args.timed_only else float(ser[dom]),
+                     "serialized_tflops": None if args.timed_only else achieved_ser,
+                     "serialized_frac": None if args.timed_only else achieved_ser / PEAK_FP64_TFLOPS,
                      "build_tflops": build_tflops, "build_frac": build_tflops / PEAK_FP64_TFLOPS},
     }
 

@@ -32,6 +32,12 @@ extern "C" {
 #define QBX_EINTERNAL (-5)
 #define QBX_ENOCONV (-6)  /* an iteration did not settle (qbx_density_purify)  */
 
+/* Shells up to g (L = 4): s, p and d quartets run the generated class
+   kernels, quartets with an f or g shell a generic kernel (always FP64).
+   QBX_SHELL_MAXC: Cartesian components of the largest shell. */
+#define QBX_MAX_L 4
+#define QBX_SHELL_MAXC 15
+
 #define QBX_PREC_FP64 0   /* every quartet in FP64                              */
 #define QBX_PREC_MIXED 1  /* FP32 primitive arithmetic for quartets whose Schwarz
                              bound is below the FP64 cut, FP64 for the rest     */
@@ -174,7 +180,8 @@ int  qbx_set_density_screening(qbx_ctx* ctx, double thr);
    angular momentum shell_L[s] and shell_nprim[s] radial primitives
    radial_xpn/radial_coef[shell_poff[s] + k] (the coefficients of its first
    component's weights); component c (Quiqbox order) is basis function
-   shell_bf[6 s + c] (-1 if absent) with weight ratio shell_sc[6 s + c].
+   shell_bf[QBX_SHELL_MAXC s + c] (-1 if absent) with weight ratio
+   shell_sc[QBX_SHELL_MAXC s + c].
    Arrays: n_shell entries (x6 for bf/sc, x3 for the centres), n_radial for the
    radial arrays; NULL entries skipped.  No reference counterpart: it exposes
    how the basis maps onto the kernels' shells, which the gradient reports in. */
@@ -229,6 +236,16 @@ int  qbx_density_purify(qbx_ctx* ctx, int32_t n, int32_t nocc, const double* X, 
 int  qbx_density_purify_noalloc(qbx_ctx* ctx, int32_t n, int32_t nocc, const double* dX, const double* dF,
                                 double* dD, int32_t max_iter, int32_t* iterations, double* idempotency,
                                 void* stream);
+
+/* One SCF step's orbitals and density on the GPU: getC + getD
+   (solveFockMatrix, HartreeFock.jl:46-59; getD :296-302).  eigen(X' F X)
+   ascending, C = X C' with each column's sign fixed so that C[0][j] >= 0,
+   D = C[:, :nocc] C[:, :nocc]^T; X' F X, X C' and D are FP64 matrix-core
+   GEMMs and the symmetric eigenproblem is rocSOLVER's divide and conquer
+   (dsyevd).  Host arrays, n x n row-major (X and F symmetric); eps (n) and
+   D may be NULL.  QBX_ENOCONV if the eigensolver does not converge. */
+int  qbx_fock_solve(qbx_ctx* ctx, int32_t n, int32_t nocc, const double* X, const double* F, double* C,
+                    double* eps, double* D);
 
 /* C[M x N] = alpha op(A) op(B) + beta C in FP64 on the matrix cores
    (v_mfma_f64_16x16x4f64), row-major device matrices, op as for

@@ -1,7 +1,7 @@
 """Build libqbx_hip.so for gfx950 in-tree.
 
 Steps: run csrc/gen_eri.py (one translation unit per ERI angular class into
-csrc/gen/), compile csrc/qbx.hip, csrc/qbx_mlp.hip, csrc/qbx_linalg.hip, csrc/qbx_boys_probe.hip and every
+csrc/gen/), compile csrc/qbx.hip, csrc/qbx_mlp.hip, csrc/qbx_linalg.hip, csrc/qbx_boys_probe.hip, csrc/qbx_eri_generic.hip and every
 csrc/gen/eri_*.hip to objects in
 parallel with hipcc, link quiqbox_amd/libqbx_hip.so.  Objects whose sources and
 headers are unchanged are reused.
@@ -75,7 +75,8 @@ def build(verbose=True, extra=(), jobs=None, force=False):
     headers = sorted(glob.glob(os.path.join(CSRC, "*.h")) + [os.path.join(GEN, "eri_table.inc"),
                                                               os.path.join(INCLUDE, "qbx.h")])
     srcs = [os.path.join(CSRC, "qbx.hip"), os.path.join(CSRC, "qbx_mlp.hip"),
-            os.path.join(CSRC, "qbx_linalg.hip"), os.path.join(CSRC, "qbx_boys_probe.hip")] + sorted(glob.glob(os.path.join(GEN, "eri_*.hip")),
+            os.path.join(CSRC, "qbx_linalg.hip"), os.path.join(CSRC, "qbx_boys_probe.hip"),
+            os.path.join(CSRC, "qbx_eri_generic.hip")] + sorted(glob.glob(os.path.join(GEN, "eri_*.hip")),
                                                      key=lambda p: -os.path.getsize(p))
     jobs = jobs or max(1, min(len(srcs), os.cpu_count() or 4))
     objs = {}
@@ -88,7 +89,9 @@ def build(verbose=True, extra=(), jobs=None, force=False):
                 print("[qbx build] %-14s %6.1f s" % (os.path.basename(futs[fu]), dt), flush=True)
             if msg.strip() and verbose:
                 print(msg, flush=True)
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", OUT + ".tmp"] + [objs[s] for s in srcs]
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(HIPCC))), "lib")
+    cmd = ([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", OUT + ".tmp"] + [objs[s] for s in srcs] +
+           ["-L" + rocm_lib, "-Wl,-rpath," + rocm_lib, "-lrocsolver", "-lrocblas"])
     subprocess.run(cmd, check=True)
     os.replace(OUT + ".tmp", OUT)
     if verbose:

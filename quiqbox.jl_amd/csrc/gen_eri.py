@@ -58,6 +58,9 @@ import os
 import sys
 
 LMAX = 2
+# shell pairs are classified up to this angular momentum (f and g shells);
+# quartets with a shell above LMAX run the generic kernel (qbx_eri_generic.hip)
+PAIR_LMAX = 4
 # classes with an adjoint (gradient) kernel: every shell s or p
 GRAD_LMAX = 1
 # classes evaluated in groups of bra components through a workspace
@@ -542,13 +545,16 @@ def stage_ws(la, lb, lc, ld):
 
 
 def class_list():
-    pcl = [(a, b) for a in range(LMAX + 1) for b in range(a + 1)]
-    order = {c: n for n, c in enumerate(pcl)}
+    """ERI classes over the pair classes up to LMAX, and every pair class up
+    to PAIR_LMAX (those up to LMAX first, in the same order)."""
+    low = [(a, b) for a in range(LMAX + 1) for b in range(a + 1)]
+    order = {c: n for n, c in enumerate(low)}
     res = []
-    for X in pcl:
-        for Y in pcl:
+    for X in low:
+        for Y in low:
             if order[X] >= order[Y]:
                 res.append((X[0], X[1], Y[0], Y[1]))
+    pcl = low + [(a, b) for a in range(LMAX + 1, PAIR_LMAX + 1) for b in range(a + 1)]
     return res, pcl
 
 
@@ -720,6 +726,10 @@ def main(outdir):
     lines = ["// Generated by gen_eri.py -- do not edit.", "#pragma once",
              "#define QBX_GEN_LMAX %d" % LMAX,
              "#define QBX_N_PAIR_CLASSES %d" % len(pcl),
+             "// pair classes 0 .. QBX_N_LOW_PAIR_CLASSES-1 (both shells up to QBX_GEN_LMAX) form the"
+             " generated ERI classes",
+             "#define QBX_N_LOW_PAIR_CLASSES %d" % ((LMAX + 1) * (LMAX + 2) // 2),
+             "#define QBX_PAIR_LMAX %d" % PAIR_LMAX,
              "#define QBX_N_CLASSES %d" % len(classes),
              "static const int qbx_pair_class_l[%d][2] = {%s};" % (
                  len(pcl), ", ".join("{%d, %d}" % c for c in pcl)),

@@ -31,6 +31,9 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "qbx_common.h"
 #include "qbx_eri_api.h"
 #include "qbx_linalg.h"
@@ -76,6 +79,8 @@ template <typename T> struct DevBuf {
 };
 
 constexpr double kPi = 3.14159265358979323846;
+static_assert(QBX_MAXC == QBX_SHELL_MAXC && QBX_MAXL == QBX_MAX_L, "shell limits of qbx.h and the kernels differ");
+static_assert(QBX_PAIR_LMAX == QBX_MAXL, "pair classes must cover every shell");
 // auxiliary streams of a Fock build: a process gets 4 hardware queues, one of
 // them the context's main stream, so a fourth auxiliary stream would share a
 // queue with another and leave its work serialized behind it
@@ -98,6 +103,7 @@ constexpr double QBX_PRIM_PAIR_CUT = 1e-20;
 // ket range per bucket.  Estimated on the (ss|ss) kets of 24 H2O in cc-pVDZ:
 // lanes busy 36% -> 78% of the primitive loop, 3.5% more waves.
 constexpr int kKetBuckets = 5;
+
 // Fixed device work array of qbx_density_purify_noalloc: P, Q, T (n x n) and
 // 2n + 4 auxiliary values for n up to QBX_PURIFY_NOALLOC_MAX_N, 25 MB per
 // GPU for the life of the process.  Host code never uses the symbol itself as
@@ -149,6 +155,7 @@ struct qbx_ctx {
                                    // one-wave-per-chunk kernels run everywhere)
     DevBuf<double> la_X, la_F, la_P, la_Q, la_T, la_aux;   // density purification
     double* scratch = nullptr;     // this device's address of the fixed work array
+    rocblas_handle blas = nullptr; // rocSOLVER's handle (qbx_fock_solve), created on first use
     std::mutex mu;
 };
 
@@ -233,6 +240,20 @@ struct qbx_basis {
     hipEvent_t ev_done = nullptr;
     hipStream_t last_stream = nullptr;
     bool built = false;
+    // shell quartets with a shell above the generated classes (f, g): the
+    // generic kernel's bra and ket lists at the threshold `thr`
+    bool has_high = false;
+    int nm = 0;                     // its recurrence size, 4 x the highest L
+    DevBuf<double> d_pexp;
+    struct HighLists {
+        bool valid = false;
+        double thr = -1.0;
+        int nbra = 0;
+        int64_t total = 0;
+        DevBuf<int> hA, hB, ho, hn, lA, lB, lo, ln, nlow;
+        DevBuf<double> hQ, lQ;
+        DevBuf<int64_t> prefix;
+    } hl;
     ~qbx_basis()
     {
         if (ev_done) (void)hipEventDestroy(ev_done);
@@ -269,8 +290,9 @@ void build_boys_table(std::vector<double>& tab, std::vector<double>& et, std::ve
     tab.assign((size_t)QBX_BOYS_ROWS * QBX_BOYS_NCOL, 0.0);
     et.assign(QBX_BOYS_ROWS, 0.0);
     rows.assign((size_t)(QBX_BOYSL_LMAX + 1) * QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE, 0.0);
-    long double F[24];
-    // one-electron kernels: F_0..F_16 on the 0.1 grid
+    long double F[32];
+    static_assert(QBX_BOYS_NCOL <= 32, "one-electron Boys table orders too high");
+    // one-electron kernels and the generic ERI kernel: F_0..F_24 on the 0.1 grid
     for (int i = 0; i < QBX_BOYS_ROWS; ++i) {
         const long double e = boys_seq_ld((long double)i * QBX_BOYS_DT, QBX_BOYS_NCOL - 1, F);
         for (int m = 0; m < QBX_BOYS_NCOL; ++m) tab[(size_t)i * QBX_BOYS_NCOL + m] = (double)F[m];
@@ -313,6 +335,7 @@ struct OneArgs {
     const int* own;        // ONE_NUCLEAR_OWN: nucleus at each shell's centre (-1: none)
     double M[3];
     int deg[3];
+    int mc;                // Cartesian components per shell in the thread layout
     double* out;
 };
 
@@ -349,11 +372,6 @@ __device__ double coulomb1_comp_t(const int* la, const int* lb, const double* PA
     return vals[0];
 }
 
-__device__ double coulomb1_comp(const int* la, const int* lb, const double* PA, const double* PC,
-                                const double* AB, double inv2p, const double* F, int Ltot)
-{
-    return coulomb1_comp_t<2 * QBX_MAXL>(la, lb, PA, PC, AB, inv2p, F, Ltot);
-}
 
 // ---------------------------------------------------------------------------
 // kernel: one-electron part of the RHF energy gradient
@@ -410,9 +428,9 @@ __device__ double one_e_energy(const int* la, const int* lb, const PrimPair& P, 
         for (int n = 0; n < a.nnuc; ++n) {
             const double PC[3] = {P.Px - a.R[3 * n], P.Py - a.R[3 * n + 1], P.Pz - a.R[3 * n + 2]};
             const double T = P.p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
-            double F[2 * QBX_MAXL + 5];
+            double F[9];   // s and p shells (the gradient rejects higher L), raised by up to two orders
             qbx_boys_rt(Lt, T, F, a.boys, a.bexp);
-            nuc += a.Z[n] * coulomb1_comp_t<2 * QBX_MAXL + 4>(la, lb, PA, PC, AB, P.inv2p, F, Lt);
+            nuc += a.Z[n] * coulomb1_comp_t<8>(la, lb, PA, PC, AB, P.inv2p, F, Lt);
         }
         nuc *= -2.0 * kPi * P.Kp;
     }
@@ -484,16 +502,19 @@ __global__ __launch_bounds__(256) void one_e_grad_kernel(OneGradArgs a)
     }
 }
 
-template <int KIND>
+// One thread per (shell pair, component pair); a.mc components per shell
+// (those of the basis's highest L).  NM: twice the highest L (the nuclear
+// attraction's recurrence size).
+template <int KIND, int NM>
 __global__ __launch_bounds__(256) void one_e_kernel(OneArgs a)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int pair = gid / (QBX_MAXC * QBX_MAXC);
-    const int slot = gid % (QBX_MAXC * QBX_MAXC);
+    const int pair = gid / (a.mc * a.mc);
+    const int slot = gid % (a.mc * a.mc);
     if (pair >= a.npair) return;
     const DevShell A = a.sh[a.pA[pair]];
     const DevShell B = a.sh[a.pB[pair]];
-    const int ca = slot / QBX_MAXC, cb = slot % QBX_MAXC;
+    const int ca = slot / a.mc, cb = slot % a.mc;
     if (ca >= qbx_ncart(A.L) || cb >= qbx_ncart(B.L)) return;
     const int ia = A.bf[ca], ib = B.bf[cb];
     if (ia < 0 || ib < 0) return;
@@ -514,9 +535,9 @@ __global__ __launch_bounds__(256) void one_e_kernel(OneArgs a)
                 if (n < 0) continue;
                 const double PC[3] = {P.Px - a.R[3 * n], P.Py - a.R[3 * n + 1], P.Pz - a.R[3 * n + 2]};
                 const double T = P.p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
-                double F[2 * QBX_MAXL + 1];
+                double F[NM + 1];
                 qbx_boys_rt(Lt, T, F, a.boys, a.bexp);
-                const double v = -2.0 * kPi * P.Kp * a.Z[n] * coulomb1_comp(la, lb, PA, PC, AB, P.inv2p, F, Lt);
+                const double v = -2.0 * kPi * P.Kp * a.Z[n] * coulomb1_comp_t<NM>(la, lb, PA, PC, AB, P.inv2p, F, Lt);
                 if (side) accB += v;
                 else acc += v;
             }
@@ -527,9 +548,9 @@ __global__ __launch_bounds__(256) void one_e_kernel(OneArgs a)
             for (int n = 0; n < a.nnuc; ++n) {
                 const double PC[3] = {Pc[0] - a.R[3 * n], Pc[1] - a.R[3 * n + 1], Pc[2] - a.R[3 * n + 2]};
                 const double T = P.p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
-                double F[2 * QBX_MAXL + 1];
+                double F[NM + 1];
                 qbx_boys_rt(Lt, T, F, a.boys, a.bexp);
-                v += a.Z[n] * coulomb1_comp(la, lb, PA, PC, AB, P.inv2p, F, Lt);
+                v += a.Z[n] * coulomb1_comp_t<NM>(la, lb, PA, PC, AB, P.inv2p, F, Lt);
             }
             acc += -2.0 * kPi * P.Kp * v;
         } else {
@@ -775,14 +796,72 @@ void find_twins(qbx_basis* b)
     }
 }
 
+bool high_class(int c) { return qbx_pair_class_l[c][0] > QBX_GEN_LMAX; }
+
+GenArgs gen_base(const qbx_basis* b)
+{
+    GenArgs g;
+    std::memset(&g, 0, sizeof(g));
+    g.sh = b->d_sh.p;
+    g.pp = b->d_pp.p;
+    g.ppk = b->d_ppk.p;
+    g.poff = b->d_poff.p;
+    g.pexp = b->d_pexp.p;
+    g.boys = b->ctx->boys.p;
+    g.bexp = b->ctx->bexp.p;
+    g.N = b->nbf;
+    return g;
+}
+
+// Schwarz bounds of the pairs of a pair class above the generated classes:
+// max |(ab|ab)| over the pair's components from the generic kernel, in the
+// class's current order.
+std::vector<double> high_class_schwarz(qbx_basis* b, int c, hipStream_t st)
+{
+    PairClass& P = b->pc[c];
+    const int n = P.n();
+    DevBuf<int64_t> pre;
+    DevBuf<unsigned long long> bits;
+    std::vector<int64_t> h(n + 1);
+    std::iota(h.begin(), h.end(), (int64_t)0);
+    pre.upload(h, st);
+    bits.ensure(n);
+    QBX_HIP(hipMemsetAsync(bits.p, 0, n * sizeof(unsigned long long), st));
+    GenArgs g = gen_base(b);
+    g.nbra = n;
+    g.total = n;
+    g.prefix = pre.p;
+    g.hA = P.d_shA.p;
+    g.hB = P.d_shB.p;
+    g.ho = P.d_ppo.p;
+    g.hn = P.d_ppn.p;
+    g.qbits = bits.p;
+    launch_eri_generic(g, 2 * (qbx_pair_class_l[c][0] + qbx_pair_class_l[c][1]), MODE_SCHWARZ, st);
+    std::vector<unsigned long long> hb(n);
+    QBX_HIP(hipMemcpyAsync(hb.data(), bits.p, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    QBX_HIP(hipStreamSynchronize(st));
+    std::vector<double> Q(n);
+    for (int i = 0; i < n; ++i) {
+        double v;
+        std::memcpy(&v, &hb[i], sizeof(double));
+        Q[i] = std::sqrt(v);
+    }
+    return Q;
+}
+
 void ensure_schwarz(qbx_basis* b)
 {
     if (b->schwarz) return;
     hipStream_t st = b->ctx->stream;
     std::vector<DevBuf<double>> q(QBX_N_PAIR_CLASSES);
+    std::vector<std::vector<double>> qhigh(QBX_N_PAIR_CLASSES);
     for (int c = 0; c < QBX_N_PAIR_CLASSES; ++c) {
         PairClass& P = b->pc[c];
         if (P.n() == 0) continue;
+        if (high_class(c)) {
+            qhigh[c] = high_class_schwarz(b, c, st);
+            continue;
+        }
         q[c].ensure(P.n());
         EriArgs a;
         std::memset(&a, 0, sizeof(a));
@@ -812,8 +891,12 @@ void ensure_schwarz(qbx_basis* b)
             continue;
         }
         std::vector<double> Q(n);
-        QBX_HIP(hipMemcpyAsync(Q.data(), q[c].p, n * sizeof(double), hipMemcpyDeviceToHost, st));
-        QBX_HIP(hipStreamSynchronize(st));
+        if (high_class(c)) {
+            Q = qhigh[c];
+        } else {
+            QBX_HIP(hipMemcpyAsync(Q.data(), q[c].p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+            QBX_HIP(hipStreamSynchronize(st));
+        }
         std::vector<int> perm(n);
         std::iota(perm.begin(), perm.end(), 0);
         std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return Q[x] > Q[y]; });
@@ -1105,6 +1188,97 @@ void ensure_tasks(qbx_basis* b, double thr, int mode, double cut)
     b->task_cut = cut;
 }
 
+// Quartet lists of the generic kernel at threshold thr: bras are the pairs of
+// the classes above the generated ones in descending Schwarz order; bra i
+// takes every lower pair with Q_i Q_j >= thr, then the higher pairs j <= i
+// with Q_i Q_j >= thr (each higher-higher quartet once).
+void ensure_hl_tasks(qbx_basis* b, double thr)
+{
+    if (!b->has_high) return;
+    auto& H = b->hl;
+    if (H.valid && H.thr == thr) return;
+    if (b->built) QBX_HIP(hipEventSynchronize(b->ev_done));
+    struct Pr {
+        int A, B, o, n;
+        double Q;
+    };
+    std::vector<Pr> hi, lo;
+    for (int c = 0; c < QBX_N_PAIR_CLASSES; ++c) {
+        const PairClass& P = b->pc[c];
+        auto& dst = high_class(c) ? hi : lo;
+        for (int i = 0; i < P.n(); ++i) dst.push_back({P.shA[i], P.shB[i], P.ppo[i], P.ppn[i], P.Q[i]});
+    }
+    auto byQ = [](const Pr& x, const Pr& y) { return x.Q > y.Q; };
+    std::stable_sort(hi.begin(), hi.end(), byQ);
+    std::stable_sort(lo.begin(), lo.end(), byQ);
+    std::vector<double> hq(hi.size()), lq(lo.size());
+    for (size_t i = 0; i < hi.size(); ++i) hq[i] = hi[i].Q;
+    for (size_t i = 0; i < lo.size(); ++i) lq[i] = lo[i].Q;
+    const int nb = (int)hi.size();
+    std::vector<int64_t> pre(nb + 1, 0);
+    std::vector<int> nlow(nb);
+    for (int i = 0; i < nb; ++i) {
+        int nl, nh;
+        if (thr <= 0.0) {
+            nl = (int)lo.size();
+            nh = i + 1;
+        } else if (hq[i] > 0.0) {
+            nl = count_ge(lq, thr / hq[i]);
+            nh = std::min(i + 1, count_ge(hq, thr / hq[i]));
+        } else {
+            nl = nh = 0;
+        }
+        nlow[i] = nl;
+        pre[i + 1] = pre[i] + nl + nh;
+    }
+    hipStream_t st = b->ctx->stream;
+    auto up = [&](const std::vector<Pr>& v, DevBuf<int>& A, DevBuf<int>& B, DevBuf<int>& o, DevBuf<int>& n,
+                  DevBuf<double>& Q, const std::vector<double>& q) {
+        std::vector<int> a(v.size()), bb(v.size()), oo(v.size()), nn(v.size());
+        for (size_t i = 0; i < v.size(); ++i) {
+            a[i] = v[i].A;
+            bb[i] = v[i].B;
+            oo[i] = v[i].o;
+            nn[i] = v[i].n;
+        }
+        A.upload(a, st);
+        B.upload(bb, st);
+        o.upload(oo, st);
+        n.upload(nn, st);
+        Q.upload(q, st);
+    };
+    up(hi, H.hA, H.hB, H.ho, H.hn, H.hQ, hq);
+    up(lo, H.lA, H.lB, H.lo, H.ln, H.lQ, lq);
+    H.nlow.upload(nlow, st);
+    H.prefix.upload(pre, st);
+    QBX_HIP(hipStreamSynchronize(st));
+    H.nbra = nb;
+    H.total = pre[nb];
+    H.thr = thr;
+    H.valid = true;
+}
+
+GenArgs gen_lists(const qbx_basis* b)
+{
+    GenArgs g = gen_base(b);
+    const auto& H = b->hl;
+    g.nbra = H.nbra;
+    g.total = H.total;
+    g.prefix = H.prefix.p;
+    g.nlow = H.nlow.p;
+    g.hA = H.hA.p;
+    g.hB = H.hB.p;
+    g.ho = H.ho.p;
+    g.hn = H.hn.p;
+    g.hQ = H.hQ.p;
+    g.lA = H.lA.p;
+    g.lB = H.lB.p;
+    g.lo = H.lo.p;
+    g.ln = H.ln.p;
+    g.lQ = H.lQ.p;
+    return g;
+}
+
 // Before a Fock build on stream st touches the basis's work buffers: a
 // previous build still running on another stream must finish first (stream
 // order covers a build on the same stream).
@@ -1173,18 +1347,32 @@ void run_one_e(qbx_ctx* ctx, const qbx_basis* b, int kind, int nnuc, const doubl
             a.deg[t] = deg[t];
         }
     }
-    const int64_t threads = (int64_t)a.npair * QBX_MAXC * QBX_MAXC;
+    a.mc = qbx_ncart(b->maxL);
+    const int64_t threads = (int64_t)a.npair * a.mc * a.mc;
     if (threads > 0) {
+        if ((threads + 255) / 256 > 0x7fffffffLL) throw QbxError(QBX_EUNSUP, "basis too large for one grid");
         const int blocks = (int)((threads + 255) / 256);
-        switch (kind) {
-        case ONE_OVERLAP: hipLaunchKernelGGL(one_e_kernel<ONE_OVERLAP>, dim3(blocks), dim3(256), 0, st, a); break;
-        case ONE_KINETIC: hipLaunchKernelGGL(one_e_kernel<ONE_KINETIC>, dim3(blocks), dim3(256), 0, st, a); break;
-        case ONE_NUCLEAR: hipLaunchKernelGGL(one_e_kernel<ONE_NUCLEAR>, dim3(blocks), dim3(256), 0, st, a); break;
-        case ONE_NUCLEAR_OWN:
-            hipLaunchKernelGGL(one_e_kernel<ONE_NUCLEAR_OWN>, dim3(blocks), dim3(256), 0, st, a);
-            break;
-        default: hipLaunchKernelGGL(one_e_kernel<ONE_MULTIPOLE>, dim3(blocks), dim3(256), 0, st, a); break;
-        }
+        auto go = [&](auto nm) {
+            constexpr int NM = decltype(nm)::value;
+            switch (kind) {
+            case ONE_OVERLAP:
+                hipLaunchKernelGGL((one_e_kernel<ONE_OVERLAP, NM>), dim3(blocks), dim3(256), 0, st, a);
+                break;
+            case ONE_KINETIC:
+                hipLaunchKernelGGL((one_e_kernel<ONE_KINETIC, NM>), dim3(blocks), dim3(256), 0, st, a);
+                break;
+            case ONE_NUCLEAR:
+                hipLaunchKernelGGL((one_e_kernel<ONE_NUCLEAR, NM>), dim3(blocks), dim3(256), 0, st, a);
+                break;
+            case ONE_NUCLEAR_OWN:
+                hipLaunchKernelGGL((one_e_kernel<ONE_NUCLEAR_OWN, NM>), dim3(blocks), dim3(256), 0, st, a);
+                break;
+            default: hipLaunchKernelGGL((one_e_kernel<ONE_MULTIPOLE, NM>), dim3(blocks), dim3(256), 0, st, a); break;
+            }
+        };
+        // recurrence tables sized for the basis: s/p/d (as before), f and g
+        if (b->maxL <= 2) go(std::integral_constant<int, 4>());
+        else go(std::integral_constant<int, 2 * QBX_MAXL>());
         QBX_HIP(hipGetLastError());
     }
     QBX_HIP(hipMemcpyAsync(out, dout.p, NN * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1218,6 +1406,7 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
 {
     ensure_schwarz(b);
     ensure_tasks(b, thr, mode, ctx->mixed_cut);
+    ensure_hl_tasks(b, thr);
     const size_t NN = (size_t)b->nbf * b->nbf;
     const int copies = ctx->xcd_local ? QBX_MAX_XCD : 1;
     b->d_J.ensure(NN * nset * copies);
@@ -1255,6 +1444,21 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
     if (fork) {
         QBX_HIP(hipEventRecord(ctx->ev_fork, st));
         for (int k = 0; k < kStreams; ++k) QBX_HIP(hipStreamWaitEvent(ctx->aux[k], ctx->ev_fork, 0));
+    }
+    // quartets with an f or g shell: the generic kernel (FP64 in every mode),
+    // first on the last auxiliary stream
+    if (b->has_high && b->hl.total > 0) {
+        GenArgs g = gen_lists(b);
+        g.D = dD;
+        g.J = b->d_J.p;
+        g.K = b->d_K.p;
+        g.xcd_local = ctx->xcd_local ? 1 : 0;
+        g.jk_stride = NN * nset;
+        g.nset = nset;
+        g.pdmax = pdmax;
+        g.nshell = ns;
+        g.dthr = ctx->density_thr;
+        launch_eri_generic(g, b->nm, MODE_JK, fork ? ctx->aux[kStreams - 1] : st);
     }
     std::vector<int> order(QBX_N_CLASSES);
     std::iota(order.begin(), order.end(), 0);
@@ -1523,6 +1727,7 @@ void qbx_finalize(qbx_ctx* ctx)
         (void)hipEventDestroy(ctx->ev_join[k]);
     }
     (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     for (auto& T : ctx->tsets) {
         for (int c = 0; c < QBX_N_CLASSES; ++c) {
             (void)hipEventDestroy(T.beg_l[c]);
@@ -1700,6 +1905,9 @@ int qbx_basis_create(qbx_ctx* ctx, int32_t n_prim, const double* prim_cen, const
             }
         }
         b->d_sh.upload(ds, st);
+        b->d_pexp.upload(b->pexp, st);
+        b->has_high = b->maxL > QBX_GEN_LMAX;
+        b->nm = 4 * b->maxL;
         b->d_pp.upload(b->pp, st);
         b->d_ppk.upload(b->ppk, st);
         std::vector<int> poff(ns);
@@ -1829,6 +2037,12 @@ int qbx_eri_dense(qbx_ctx* ctx, const qbx_basis* bc, double* eri)
             k_dense[cls](a, st);
             QBX_HIP(hipStreamSynchronize(st));
         }
+        if (b->has_high) {
+            ensure_hl_tasks(b, 0.0);
+            GenArgs g = gen_lists(b);
+            g.eri = d.p;
+            launch_eri_generic(g, b->nm, MODE_DENSE, st);
+        }
         QBX_HIP(hipMemcpyAsync(eri, d.p, N4 * sizeof(double), hipMemcpyDeviceToHost, st));
         QBX_HIP(hipStreamSynchronize(st));
     });
@@ -1928,6 +2142,71 @@ int qbx_density_purify_noalloc(qbx_ctx* ctx, int32_t n, int32_t nocc, const doub
         const PurifyWork w{base, base + nn, base + 2 * nn, base + 3 * nn};
         const double* dOut = purify_density(n, nocc, dX, dF, w, max_iter, iterations, idempotency, st);
         QBX_HIP(hipMemcpyAsync(dD, dOut, nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+        QBX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int qbx_fock_solve(qbx_ctx* ctx, int32_t n, int32_t nocc, const double* X, const double* F, double* C, double* eps,
+                   double* D)
+{
+    return guarded([&] {
+        if (!ctx) throw QbxError(QBX_EINVAL, "null context");
+        if (n <= 0 || nocc < 0 || nocc > n) throw QbxError(QBX_EINVAL, "need 0 <= nocc <= n and n > 0");
+        if (!X || !F || !C) throw QbxError(QBX_EINVAL, "null matrix pointer");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        QBX_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        if (!ctx->blas && rocblas_create_handle(&ctx->blas) != rocblas_status_success) {
+            ctx->blas = nullptr;
+            throw QbxError(QBX_EINTERNAL, "rocblas_create_handle failed");
+        }
+        if (rocblas_set_stream(ctx->blas, st) != rocblas_status_success)
+            throw QbxError(QBX_EINTERNAL, "rocblas_set_stream failed");
+        const size_t nn = (size_t)n * n;
+        ctx->la_X.ensure(nn);
+        ctx->la_F.ensure(nn);
+        ctx->la_P.ensure(nn);
+        ctx->la_Q.ensure(nn);
+        ctx->la_T.ensure(nn);
+        ctx->la_aux.ensure(3 * (size_t)n + 4);
+        double* dX = ctx->la_X.p;
+        double* dF = ctx->la_F.p;
+        double* dP = ctx->la_P.p;
+        double* dQ = ctx->la_Q.p;
+        double* dT = ctx->la_T.p;
+        double* dE = ctx->la_aux.p;             // eigenvalues (n)
+        double* dOff = dE + n;                  // off-diagonal work of the tridiagonal form (n)
+        double* dSgn = dOff + n;                // column signs (n)
+        rocblas_int* dInfo = (rocblas_int*)(dSgn + n);
+        QBX_HIP(hipMemcpyAsync(dX, X, nn * sizeof(double), hipMemcpyHostToDevice, st));
+        QBX_HIP(hipMemcpyAsync(dF, F, nn * sizeof(double), hipMemcpyHostToDevice, st));
+        // Fock matrix in the orthonormal basis: X' F X, symmetrized
+        gemm_f64(false, false, n, n, n, 1.0, dF, n, dX, n, 0.0, nullptr, 0, dT, n, st);
+        gemm_f64(true, false, n, n, n, 1.0, dX, n, dT, n, 0.0, nullptr, 0, dP, n, st);
+        symmetrize_square(dP, n, st);
+        // eigenpairs, ascending; the column-major eigenvector matrix V read
+        // row-major is V^T
+        if (rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, n, dP, n, dE, dOff, dInfo) !=
+            rocblas_status_success)
+            throw QbxError(QBX_EINTERNAL, "rocsolver_dsyevd failed");
+        rocblas_int info = 0;
+        QBX_HIP(hipMemcpyAsync(&info, dInfo, sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+        QBX_HIP(hipStreamSynchronize(st));
+        if (info != 0) throw QbxError(QBX_ENOCONV, "the symmetric eigensolver did not converge");
+        // C = X V with every column's first entry made non-negative
+        gemm_f64(false, true, n, n, n, 1.0, dX, n, dP, n, 0.0, nullptr, 0, dQ, n, st);
+        fix_column_signs(dQ, n, dSgn, st);
+        QBX_HIP(hipMemcpyAsync(C, dQ, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (eps) QBX_HIP(hipMemcpyAsync(eps, dE, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (D) {
+            // D = C[:, :nocc] C[:, :nocc]^T
+            if (nocc > 0) {
+                gemm_f64(false, true, n, n, nocc, 1.0, dQ, n, dQ, n, 0.0, nullptr, 0, dT, n, st);
+                QBX_HIP(hipMemcpyAsync(D, dT, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+            } else {
+                std::memset(D, 0, nn * sizeof(double));
+            }
+        }
         QBX_HIP(hipStreamSynchronize(st));
     });
 }

@@ -17,8 +17,9 @@
 #define QBX_BOYS_INVDT 10.0
 #define QBX_BOYS_TMAX 60.0
 #define QBX_BOYS_ROWS 602
-#define QBX_BOYS_NCOL 17     // F_0 .. F_16: highest order 8 plus 8 Taylor terms
+#define QBX_BOYS_NCOL 25     // F_0 .. F_24: highest order 17 (g quartets need 16) plus 7 Taylor orders above it
 #define QBX_BOYS_TAYLOR 8
+#define QBX_BOYS_NMAX (QBX_BOYS_NCOL - QBX_BOYS_TAYLOR)
 // Per-order rows for the ERI kernels (staged in LDS, 23 KB): on the grid
 // T_i = i/4 (T < QBX_BOYS_TMAX), for total angular momentum L, row i holds
 // F_{L+k}(T_i) / k! for k = 0..10 and exp(-T_i) (96-byte rows).  With
@@ -30,8 +31,8 @@
 #define QBX_BOYSL_STRIDE 12
 #define QBX_BOYSL_LMAX 9
 
-#define QBX_MAXL 2           // largest shell angular momentum in the kernels
-#define QBX_MAXC 6           // Cartesian components of the largest shell
+#define QBX_MAXL 4           // largest shell angular momentum (g); above 2 the ERIs take the generic kernel
+#define QBX_MAXC 15          // Cartesian components of the largest shell
 #define QBX_GRAD_MAXP 8      // radial primitives per shell in the gradient kernels
 
 struct DevShell {
@@ -87,7 +88,9 @@ __device__ __forceinline__ void qbx_boys(double T, double* __restrict__ F,
     }
 }
 
-// Runtime-order Boys function (1e kernels): F_0..F_n, n <= 2*QBX_MAXL.
+// Runtime-order Boys function (1e kernels and the generic ERI kernel):
+// F_0..F_n for n <= QBX_BOYS_NMAX (the top order's Taylor series reads table
+// columns n .. n + QBX_BOYS_TAYLOR - 1).
 __device__ __forceinline__ void qbx_boys_rt(int n, double T, double* __restrict__ F,
                                             const double* __restrict__ tab,
                                             const double* __restrict__ etab)

@@ -979,7 +979,10 @@ __device__ __forceinline__ void staged_chunk(const EriArgs& a, int tb, int chunk
 
     // ---- J/K digest from the (a b| rows, in two passes that each keep
     //      few sums live: A) J_ab, K_ac, K_ad per row; B) J_cd summed over
-    //      all rows, K_bd and K_bc summed over a ----
+    //      all rows, K_bd and K_bc summed over a.  The density elements a
+    //      lane reads are loaded once per quartet (its D_cd block), per B
+    //      component (D_bc, D_bd) or per row (D_ac, D_ad) rather than once per
+    //      integral ----
     double f = live ? 1.0 : 0.0;
     if (sA == sB) f *= 0.5;
     if (sC == sD) f *= 0.5;
@@ -994,34 +997,47 @@ __device__ __forceinline__ void staged_chunk(const EriArgs& a, int tb, int chunk
         const double* __restrict__ DJ = a.D + (size_t)s * N * N;
         const double* __restrict__ DK = a.D + (size_t)(a.nset + s) * N * N;
         double* Js = sink.J + (size_t)s * N * N;
+        double dcd[NCD];
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = DJ[fc[ic] * N + fd[id]];
 #pragma unroll 1
-        for (int iab = 0; iab < NA * NB; ++iab) {
-            const int ia = iab / NB, ib = iab - ia * NB;
-            R out[NCD];
-            Cls::hrr_bra_ab(iab, ws, out, ABx, ABy, ABz);
-            const int fa_ = max(SA->bf[ia], 0), fb_ = max(SB->bf[ib], 0);
-            const double sab = SA->sc[ia] * SB->sc[ib];
-            double jab = 0.0, kac[NC], kad[ND];
+        for (int ib = 0; ib < NB; ++ib) {
+            const int fb_ = max(SB->bf[ib], 0);
+            double dbc[NC], dbd[ND];
 #pragma unroll
-            for (int n = 0; n < NC; ++n) kac[n] = 0.0;
+            for (int ic = 0; ic < NC; ++ic) dbc[ic] = DK[fb_ * N + fc[ic]];
 #pragma unroll
-            for (int n = 0; n < ND; ++n) kad[n] = 0.0;
+            for (int id = 0; id < ND; ++id) dbd[id] = DK[fb_ * N + fd[id]];
+#pragma unroll 1
+            for (int ia = 0; ia < NA; ++ia) {
+                R out[NCD];
+                Cls::hrr_bra_ab(ia * NB + ib, ws, out, ABx, ABy, ABz);
+                const int fa_ = max(SA->bf[ia], 0);
+                const double sab = SA->sc[ia] * SB->sc[ib];
+                double jab = 0.0, kac[NC], kad[ND];
 #pragma unroll
-            for (int ic = 0; ic < NC; ++ic)
+                for (int n = 0; n < NC; ++n) kac[n] = 0.0;
 #pragma unroll
-                for (int id = 0; id < ND; ++id) {
-                    const double v = (double)out[ic * ND + id] * sab * (scc[ic] * scdd[id]);
-                    jab += v * DJ[fc[ic] * N + fd[id]];
-                    kac[ic] += v * DK[fb_ * N + fd[id]];
-                    kad[id] += v * DK[fb_ * N + fc[ic]];
+                for (int n = 0; n < ND; ++n) kad[n] = 0.0;
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) {
+                        const double v = (double)out[ic * ND + id] * sab * (scc[ic] * scdd[id]);
+                        jab += v * dcd[ic * ND + id];
+                        kac[ic] += v * dbd[id];
+                        kad[id] += v * dbc[ic];
+                    }
+                jab = wave_sum(fj * jab);
+                if (lane == 0) sink.add(Js + fa_ * N + fb_, jab);
+                if (live) {
+#pragma unroll
+                    for (int ic = 0; ic < NC; ++ic) kout.add(s, fa_, ia, fc[ic], fk * kac[ic]);
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) kout.add(s, fa_, ia, fd[id], fk * kad[id]);
                 }
-            jab = wave_sum(fj * jab);
-            if (lane == 0) sink.add(Js + fa_ * N + fb_, jab);
-            if (live) {
-#pragma unroll
-                for (int ic = 0; ic < NC; ++ic) kout.add(s, fa_, ia, fc[ic], fk * kac[ic]);
-#pragma unroll
-                for (int id = 0; id < ND; ++id) kout.add(s, fa_, ia, fd[id], fk * kad[id]);
             }
         }
         double jcd[NCD];
@@ -1042,6 +1058,11 @@ __device__ __forceinline__ void staged_chunk(const EriArgs& a, int tb, int chunk
                 const int fa_ = max(SA->bf[ia], 0);
                 const double sab = SA->sc[ia] * SB->sc[ib];
                 const double dab = cst(DJ)[fa_ * N + fb_];
+                double dac[NC], dad[ND];
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic) dac[ic] = DK[fa_ * N + fc[ic]];
+#pragma unroll
+                for (int id = 0; id < ND; ++id) dad[id] = DK[fa_ * N + fd[id]];
 #pragma unroll
                 for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
@@ -1049,8 +1070,8 @@ __device__ __forceinline__ void staged_chunk(const EriArgs& a, int tb, int chunk
                         const int cd = ic * ND + id;
                         const double v = (double)out[cd] * sab * (scc[ic] * scdd[id]);
                         jcd[cd] += v * dab;
-                        kbd[id] += v * DK[fa_ * N + fc[ic]];
-                        kbc[ic] += v * DK[fa_ * N + fd[id]];
+                        kbd[id] += v * dac[ic];
+                        kbc[ic] += v * dad[id];
                     }
             }
             if (live) {

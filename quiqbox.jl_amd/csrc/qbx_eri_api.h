@@ -109,6 +109,50 @@ void launch_eri_grad(const EriArgs& a, hipStream_t s);
 
 typedef void (*EriLaunch)(const EriArgs&, hipStream_t);
 
+// Shell quartets with a shell above the generated classes (f and g): the
+// generic kernel of qbx_eri_generic.hip.  Bras are the pairs of the higher
+// pair classes in descending Schwarz order; the kets of bra i are the first
+// nlow[i] of the lower pairs (descending Schwarz order) followed by the first
+// prefix[i+1] - prefix[i] - nlow[i] higher pairs (those up to bra i itself).
+// One wave per quartet, lanes over its Cartesian component quartets.
+struct GenArgs {
+    const DevShell* sh;
+    const PrimPair* pp;
+    const int* ppk;           // radial indices (k_a | k_b << 16) of every primitive pair
+    const int* poff;          // first radial primitive of every shell
+    const double* pexp;       // radial exponents
+    const double* boys;       // one-electron Boys table (F_0..F_24 on the 0.1 grid)
+    const double* bexp;
+    int nbra;
+    int64_t total;            // quartets (waves)
+    const int64_t* prefix;    // [nbra + 1]
+    const int* nlow;          // [nbra]
+    const int *hA, *hB, *ho, *hn;  // higher pairs: shells, primitive-pair offset and count
+    const double* hQ;
+    const int *lA, *lB, *lo, *ln;  // lower pairs
+    const double* lQ;
+    int N;
+    // J/K mode (as EriArgs)
+    const double* D;
+    double* J;
+    double* K;
+    int xcd_local;
+    size_t jk_stride;
+    int nset;
+    const double* pdmax;
+    int nshell;
+    double dthr;
+    // dense mode
+    double* eri;
+    // Schwarz mode: every bra with itself, max |(ab|ab)| as the bits of a
+    // non-negative double
+    unsigned long long* qbits;
+};
+
+// nm: the per-axis recurrence size (4 x the highest shell angular momentum
+// of the quartets, 12 or 16); mode: MODE_JK, MODE_DENSE or MODE_SCHWARZ
+void launch_eri_generic(const GenArgs& a, int nm, int mode, hipStream_t s);
+
 // The ERI kernels' Boys function alone (qbx_boys_probe.hip): F_0..F_L at n
 // device arguments T from the order-L table rows, into F (n x (L+1)).
 void boys_probe(const double* rows, int L, const double* T, int n, double* F, hipStream_t s);

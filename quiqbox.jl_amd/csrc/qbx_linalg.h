@@ -18,6 +18,13 @@ void pair_columns(const double* C, int nao, int nmo, double* out, hipStream_t s)
 // lohi[2i] = P_ii - sum_j!=i |P_ij|, lohi[2i+1] = P_ii + sum_j!=i |P_ij|
 void gershgorin(const double* P, int n, double* lohi, hipStream_t s);
 
+// P <- (P + P^T) / 2 for the n x n matrix P
+void symmetrize_square(double* P, int n, hipStream_t s);
+
+// flip the columns j of the n x n matrix C whose first-row entry C[0][j] is
+// negative (sgn: n doubles of work space)
+void fix_column_signs(double* C, int n, double* sgn, hipStream_t s);
+
 // out[0] = min of lohi[2i], out[1] = max of lohi[2i+1] over the n rows
 void spectral_interval(const double* lohi, int n, double* out, hipStream_t s);
 

@@ -176,6 +176,32 @@ __global__ __launch_bounds__(1024) void trace_frob_kernel(const double* P, int n
     }
 }
 
+// P <- (P + P^T) / 2 in place, one thread per (i, j) with i < j
+__global__ void symmetrize_square_kernel(double* P, int n)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    const size_t i = e / n, j = e % n;
+    if (i >= j) return;
+    const double v = 0.5 * (P[i * n + j] + P[j * n + i]);
+    P[i * n + j] = v;
+    P[j * n + i] = v;
+}
+
+// column signs: sgn[j] = -1 where C[0][j] < 0, else 1
+__global__ void column_sign_kernel(const double* C, int n, double* sgn)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) sgn[j] = C[j] < 0.0 ? -1.0 : 1.0;
+}
+
+__global__ void scale_columns_kernel(double* C, int n, const double* sgn)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    C[e] *= sgn[e % n];
+}
+
 // out[0] = min_i lohi[2i], out[1] = max_i lohi[2i+1]: the spectral interval
 // from the rows' Gershgorin intervals (one block; min and max are exact in
 // any order)
@@ -252,6 +278,22 @@ void pair_columns(const double* C, int nao, int nmo, double* out, hipStream_t s)
 void gershgorin(const double* P, int n, double* lohi, hipStream_t s)
 {
     hipLaunchKernelGGL(gershgorin_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P, n, lohi);
+    QBX_HIP(hipGetLastError());
+}
+
+void symmetrize_square(double* P, int n, hipStream_t s)
+{
+    const size_t nn = (size_t)n * n;
+    hipLaunchKernelGGL(symmetrize_square_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, P, n);
+    QBX_HIP(hipGetLastError());
+}
+
+void fix_column_signs(double* C, int n, double* sgn, hipStream_t s)
+{
+    const size_t nn = (size_t)n * n;
+    hipLaunchKernelGGL(column_sign_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, C, n, sgn);
+    QBX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(scale_columns_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, C, n, sgn);
     QBX_HIP(hipGetLastError());
 }
 

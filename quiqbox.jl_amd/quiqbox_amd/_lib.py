@@ -10,6 +10,8 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libqbx_hip.so")
 
+QBX_SHELL_MAXC = 15     # Cartesian components of the largest shell (g), qbx.h
+QBX_MAX_L = 4
 QBX_PREC_FP64 = 0
 QBX_PREC_MIXED = 1
 
@@ -49,6 +51,7 @@ SIGNATURES = {
     "qbx_basis_shells": (_i32, [_vp, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd]),
     "qbx_rhf_gradient": (_i32, [_vp, _vp, _pd, _pd, _i32, _pd, _pd, _d, _pd, _pd, _pd]),
     "qbx_density_purify": (_i32, [_vp, _i32, _i32, _pd, _pd, _pd, _i32, _pi, _pd]),
+    "qbx_fock_solve": (_i32, [_vp, _i32, _i32, _pd, _pd, _pd, _pd, _pd]),
     "qbx_density_purify_noalloc": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp, _i32, _pi, _pd, _vp]),
     "qbx_mo_transform": (_i32, [_vp, _i32, _i32, _i32, _pd, _pd, _pd]),
     "qbx_mo_coulomb_ab": (_i32, [_vp, _i32, _i32, _i32, _pd, _pd, _pd, _pd]),

@@ -80,6 +80,24 @@ class Context:
         return D, int(it.value), float(idem.value)
 
 
+    def fock_solve(self, X, F, nocc):
+        """One SCF step's orbitals on the GPU (qbx_fock_solve): eigen(X' F X)
+        ascending, C = X C' with C[0, j] >= 0, and D = C[:, :nocc] C[:, :nocc]^T.
+        Returns (C, orbital energies, D)."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        n = X.shape[0]
+        if X.shape != (n, n) or F.shape != (n, n):
+            raise ValueError("X and F must be square matrices of one size")
+        C = np.empty((n, n))
+        e = np.empty(n)
+        D = np.empty((n, n))
+        pd = ctypes.POINTER(ctypes.c_double)
+        with self.lock:
+            _lib.check(_lib.load().qbx_fock_solve(self.h, n, int(nocc), X.ctypes.data_as(pd), F.ctypes.data_as(pd),
+                                                  C.ctypes.data_as(pd), e.ctypes.data_as(pd), D.ctypes.data_as(pd)))
+        return C, e, D
+
     def mo_transform(self, ints, C):
         """changeOrbitalBasis (Interface.jl:376-380) of a one-body (2-index)
         or two-body (4-index) AO tensor with the n_ao x n_mo matrix C: one
@@ -275,13 +293,14 @@ class GPUBasis:
                                       None, None))
         ns, nr = ns.value, nr.value
         Ls, npr, poff = (np.zeros(ns, dtype=np.int32) for _ in range(3))
-        bf = np.zeros(6 * ns, dtype=np.int32)
-        sc = np.zeros(6 * ns)
+        mc = _lib.QBX_SHELL_MAXC
+        bf = np.zeros(mc * ns, dtype=np.int32)
+        sc = np.zeros(mc * ns)
         cen = np.zeros(3 * ns)
         xpn, coef = np.zeros(nr), np.zeros(nr)
         _lib.check(L.qbx_basis_shells(self.h, None, None, _lib.ip(Ls), _lib.ip(npr), _lib.ip(poff), _lib.ip(bf),
                                       _lib.dp(sc), _lib.dp(cen), _lib.dp(xpn), _lib.dp(coef)))
-        return dict(L=Ls, nprim=npr, poff=poff, bf=bf.reshape(ns, 6), sc=sc.reshape(ns, 6), center=cen.reshape(ns, 3),
+        return dict(L=Ls, nprim=npr, poff=poff, bf=bf.reshape(ns, mc), sc=sc.reshape(ns, mc), center=cen.reshape(ns, 3),
                     xpn=xpn, coef=coef)
 
     def rhf_gradient(self, Dt, W, Z, R, thr=1e-12):

@@ -284,12 +284,34 @@ class PurifiedDensity:
         return 0.5 * (D + D.T)
 
 
+class DeviceEigen:
+    """getC + getD of the SCF steps on the GPU (Context.fock_solve): X' F X
+    and X C' on the FP64 matrix cores, the symmetric eigenproblem by rocSOLVER;
+    the same matrices as solveFockMatrix/getD up to rounding."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.calls = 0
+
+    def __call__(self, X, F, N):
+        self.calls += 1
+        C, _, D = self.ctx.fock_solve(X, F, N)
+        return C, D
+
+
 def getCDFE(Hcore, twoBody, X, Ns, F):
     """One SCF step: F -> C -> D -> F -> E (HartreeFock.jl:392-403).  With a
     density solver on the two-body operator (runHartreeFock's
-    densitySolver="purify") D comes from it and C is formed only if used."""
+    densitySolver="purify") D comes from it and C is formed only if used;
+    with an eigen solver ("eigen", the default with the GPU engine) C and D come
+    from the device."""
     solver = getattr(twoBody, "density_solver", None)
-    if solver is None:
+    eig = getattr(twoBody, "eigen_solver", None)
+    if solver is None and eig is not None:
+        CD = [eig(X, f, n) for f, n in zip(F, Ns)]
+        Cn = tuple(c for c, _ in CD)
+        Dn = tuple(d for _, d in CD)
+    elif solver is None:
         Cn = tuple(getC(X, f) for f in F)
         Dn = tuple(getD(c, n) for c, n in zip(Cn, Ns))
     else:
@@ -1013,12 +1035,15 @@ def runHartreeFock(nucInfo, basis, config=None, *, printInfo=True, infoLevel=def
     GPUBasis.  Returns HFfinalInfo with energy = (E_HF electronic, E_nuclear).
     Beyond the reference: `precision` (QBX_PREC_FP64 / QBX_PREC_MIXED) and
     `incremental` (density-screened incremental Fock builds) select the GPU
-    Fock-build mode; `densitySolver` "purify" forms each step's density by
-    purification on the GPU instead of diagonalization (the same matrix; the
-    coefficients of the result still come from one diagonalization).  The
-    defaults reproduce the reference's full FP64 builds and eigen steps."""
-    if densitySolver not in ("eigen", "purify"):
-        raise ValueError("densitySolver must be 'eigen' or 'purify'")
+    Fock-build mode; `densitySolver` selects each step's orbitals and density:
+    "eigen" (default) diagonalizes X' F X as the reference does, on the GPU
+    when the Fock build is (matrix-core GEMMs, rocSOLVER eigensolver),
+    "eigen-host" the same on the host (numpy), "purify" forms the density by
+    purification on the GPU instead (the same matrix; the coefficients of the
+    result still come from one diagonalization).  The defaults reproduce the
+    reference's full FP64 builds and eigen steps."""
+    if densitySolver not in ("eigen", "eigen-host", "purify"):
+        raise ValueError("densitySolver must be 'eigen', 'eigen-host' or 'purify'")
     t0 = time.time()
     if isinstance(nucInfo, tuple) and len(nucInfo) == 2 and isinstance(nucInfo[1], NuclearCluster):
         spin, nuc = nucInfo
@@ -1046,10 +1071,14 @@ def runHartreeFock(nucInfo, basis, config=None, *, printInfo=True, infoLevel=def
     hamil, C0, gb = initializeHartreeFock(config.initial, basis, system, twoBody, schwarzThreshold, precision,
                                           incremental)
     solver = None
+    eig = None
     if densitySolver == "purify":
         if not isinstance(hamil.twoBody, DirectJK):
             raise ValueError("densitySolver='purify' runs with the GPU Fock build")
         solver = hamil.twoBody.density_solver = PurifiedDensity(gb.ctx)
+    elif (densitySolver == "eigen" and isinstance(hamil.twoBody, DirectJK) and
+          hasattr(getattr(gb, "ctx", None), "fock_solve")):
+        eig = hamil.twoBody.eigen_solver = DeviceEigen(gb.ctx)
     t1 = time.time()
     if printInfo and infoLevel > 1:
         (log or print)("Hartree–Fock (HF) Initialization:\n•HF Type: %s\n•Basis Set Size: %d\n"
@@ -1070,6 +1099,9 @@ def runHartreeFock(nucInfo, basis, config=None, *, printInfo=True, infoLevel=def
         hamil.twoBody.density_solver = None
         timing.update(purified_steps=solver.calls - solver.fallbacks, purify_fallbacks=solver.fallbacks,
                       purify_products=solver.products)
+    if eig is not None:
+        hamil.twoBody.eigen_solver = None
+        timing.update(device_eigen_steps=eig.calls)
     res = HFfinalInfo(system, (Eel, Enuc), Cfin, tuple(tv.Ds[-1] for tv in vars_),
                       Fs, occu, vars_, converged, gb, timing)
     if printInfo:

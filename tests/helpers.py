@@ -94,3 +94,19 @@ def random_spd_basis(seed, n_atoms=4, box=4.0, lmax=2, nprim=(3, 3, 2)):
                 cons = [cc * qb.get3DimPGTOrbNormFactor(x, ijk) for x, cc in zip(xp, co)]
                 bs.append(qb.genGaussTypeOrb(c, list(xp), cons, ijk, outerRenormalize=True))
     return coords, bs
+
+
+def oracle_eri_dense(ob, n, threads=16):
+    """The oracle's dense ERI tensor from its unique quartets (i >= j, k >= l,
+    ij >= kl) evaluated on several threads, scattered 8-fold."""
+    ii, jj = np.tril_indices(n)
+    m = len(ii)
+    P, Q = np.tril_indices(m)
+    q = np.stack([ii[P], jj[P], ii[Q], jj[Q]], axis=1).astype(np.int32)
+    v = ob.eri_list(q, threads=threads)
+    E = np.zeros((n, n, n, n))
+    i, j, k, l = q.T
+    for a, b, c, d in ((i, j, k, l), (j, i, k, l), (i, j, l, k), (j, i, l, k),
+                       (k, l, i, j), (l, k, i, j), (k, l, j, i), (l, k, j, i)):
+        E[a, b, c, d] = v
+    return E

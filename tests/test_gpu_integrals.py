@@ -77,7 +77,7 @@ def test_interface_functions_and_kinetic_literal():
     H = qb.coreHamiltonian(L["nucs"], L["coords"], [cgf1, cgf2])
     np.testing.assert_allclose(H, qb.nucAttractions(nuc, [cgf1, cgf2]) + qb.elecKinetics([cgf1, cgf2]),
                                rtol=0, atol=1e-14)
-    # the (1,2,2) literal (L = 5) is above the GPU build's shell limit (L <= 2): a clean
+    # the (1,2,2) literal (L = 5) is above the GPU build's shell limit (L <= 4): a clean
     # QBX_EUNSUP error; the oracle pins the value on the CPU (test_oracle.py)
     K = literals()["kinetic"]["contracted"]
     with pytest.raises(qb._lib.QbxError, match="above the build limit"):

@@ -125,3 +125,58 @@ def test_uhf_with_purified_density_matches_eigen_steps():
     _say("  E eigen %.12f purify %.12f" % (ref.energy[0], res.energy[0]))
     assert ref.converged and res.converged
     assert abs(res.energy[0] - ref.energy[0]) < 1e-10
+
+
+@pytest.mark.parametrize("system", ["water24_ccpvdz", "h2o2_631g"])
+def test_fock_solve_matches_host_eigen_step(system):
+    """qbx_fock_solve (X' F X and X C' on the matrix cores, rocSOLVER dsyevd)
+    against the host step solveFockMatrix/getD (numpy eigh): orbital
+    energies, column signs (C[0, j] >= 0), coefficients of nondegenerate
+    orbitals and the occupied density."""
+    from quiqbox_amd.hartree_fock import solveFockMatrix
+    from quiqbox_amd.systems import water_box
+    if system == "water24_ccpvdz":
+        nuc, bs = water_box(24, 0, basis="cc-pVDZ")
+    else:
+        L = literals()["h2o2_631g"]
+        coords = [tuple(L["angstrom_to_bohr"] * x for x in c) for c in L["coords_angstrom"]]
+        nuc = qb.NuclearCluster(L["nucs"], coords)
+        bs = molecule_basis(nuc.symbols, nuc.coords, "6-31G")
+    gb = qb.GPUBasis(bs)
+    X = gb.orthonormalization()
+    H = gb.core_hamiltonian(nuc.charges, nuc.coord_array)
+    nocc = int(round(sum(nuc.charges))) // 2
+    # a Fock-like matrix: the core Hamiltonian plus the two-electron part of its guess density
+    D0 = getD(getC(X, H), nocc)
+    F = H + gb.jk(2 * D0, D0, 1e-12)
+    C_ref, e_ref = solveFockMatrix(X, F)
+    D_ref = getD(C_ref, nocc)
+    C, e, D = default_context().fock_solve(X, F, nocc)
+    err_e = float(np.max(np.abs(e - e_ref)))
+    err_D = float(np.max(np.abs(D - D_ref)))
+    gaps = np.minimum(np.diff(e_ref, prepend=-np.inf), np.diff(e_ref, append=np.inf))
+    nondeg = gaps > 1e-6
+    err_C = float(np.max(np.abs(C[:, nondeg] - C_ref[:, nondeg])))
+    _say("%s (n %d): max|de| %.2e, max|dD| %.2e, max|dC| (nondegenerate) %.2e" % (system, gb.n, err_e, err_D, err_C))
+    assert np.all(C[0, :] >= 0)
+    assert err_e < 1e-11 * max(1.0, float(np.max(np.abs(e_ref))))
+    assert err_D < 1e-10
+    assert err_C < 1e-8
+
+
+def test_scf_default_device_eigen_matches_host_eigen():
+    """runHartreeFock's default step on the GPU (densitySolver="eigen") against
+    the same SCF with the host eigen step: same energy to 1e-10 Eh, same
+    number of Fock builds (the trajectory only differs by rounding)."""
+    L = literals()["h2o2_631g"]
+    coords = [tuple(L["angstrom_to_bohr"] * x for x in c) for c in L["coords_angstrom"]]
+    nuc = qb.NuclearCluster(L["nucs"], coords)
+    gb = qb.GPUBasis(molecule_basis(nuc.symbols, nuc.coords, "6-31G"))
+    cfg = qb.HFconfig(initial="CoreH", strategy=qb.SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-10)), maxStep=200)
+    dev = qb.runHartreeFock(nuc, gb, cfg, printInfo=False)
+    host = qb.runHartreeFock(nuc, gb, cfg, printInfo=False, densitySolver="eigen-host")
+    _say("H2O2 SCF: device eigen E %.12f (%d device steps), host eigen E %.12f" % (
+        dev.energy[0], dev.timing.get("device_eigen_steps", 0), host.energy[0]))
+    assert dev.converged and host.converged
+    assert dev.timing.get("device_eigen_steps", 0) > 0
+    assert abs(dev.energy[0] - host.energy[0]) < 1e-10
