@@ -66,6 +66,7 @@ GRAD_LMAX = 1
 # classes evaluated in groups of bra components through a workspace
 STAGED = {(2, 1, 1, 1), (2, 1, 2, 1), (2, 2, 1, 1), (2, 2, 2, 0), (2, 2, 2, 1), (2, 2, 2, 2)}
 STAGE_ACC = 32     # accumulators per group
+BRA_BATCH = 24     # workspace values per batch of loads in the bra transfer of a staged class
 COST_PRIM0 = 80    # launch-order cost model: geometry and Boys values per primitive quartet
 COST_DIGEST = 8    # launch-order cost model: J/K digest per Cartesian quartet
 
@@ -78,6 +79,13 @@ def ncart_prod(c):
 # instead of 31.  Measured on the 592-function cluster: (dd|dd) 4.1 -> 2.7 ms;
 # the other staged classes lose more latency hiding than they save.
 STAGE_ONE_WAVE = {(2, 2, 2, 2): 48}
+# Staged classes whose groups of one wave task are spread over the waves of
+# a block (value: waves per task).  (dd|dd) has too few tasks (12470 shell
+# quartets on the 592-function cluster) to give every SIMD more than one
+# wave; four waves per task measured 2.78 -> 1.25 ms.  (dd|dp) at eight waves
+# per task: 4.45 -> 4.38 ms alone, but its whole-CU blocks waited behind the
+# other classes of the concurrent build (7.7 -> 25.7 ms there).
+STAGE_SPLIT = {(2, 2, 2, 2): 4}
 TWIN_MAX_ACC = 9   # must match qbx_eri_api.h
 
 
@@ -219,6 +227,7 @@ def gen_staged(la, lb, lc, ld, es, fs, eidx, fidx, C_, D_):
     code.append("    static constexpr bool STAGED = true;")
     code.append("    static constexpr int WAVES_PER_SIMD = %d;" % (1 if (la, lb, lc, ld) in STAGE_ONE_WAVE else 2))
     code.append("    static constexpr int NGRP = %d;" % G)
+    code.append("    static constexpr int SPLIT = %d;" % STAGE_SPLIT.get((la, lb, lc, ld), 1))
     code.append("    static constexpr int NCD = %d;" % NCD)
     code.append("    // group g holds bra components e in [ge0(g), ge0(g + 1))")
     sel = " : ".join("g == %d ? %d" % (k, bounds[k]) for k in range(G)) + " : %d" % nE
@@ -307,14 +316,34 @@ def gen_staged(la, lb, lc, ld, es, fs, eidx, fidx, C_, D_):
                     names[(mono, c)] = nm
                     expr = mono_expr(mono)
                     pre.append("const R %s = (R)%d * %s;" % (nm, c, expr) if c != 1 else "const R %s = %s;" % (nm, expr))
+            # The row's workspace values are loaded in batches of about
+            # BRA_BATCH values (CB kets cd x the row's terms), each batch
+            # issued before any of it is used: left to itself the scheduler
+            # of these register-bound kernels waits for every load before
+            # issuing the next, one memory latency per value.
+            nt = len(terms)
+            cb = max(1, min(NCD, BRA_BATCH // nt))
             sum_terms = []
-            for (ei, mono, c) in terms:
-                w = "ws.ld(%d * NCD + cd)" % ei
+            for t, (ei, mono, c) in enumerate(terms):
+                w = "w[%d][j]" % t
                 sum_terms.append(w if (sum(mono) == 0 and c == 1) else "%s * %s" % (names[(mono, c)], w))
             flops_bra += NCD * (2 * len(terms) - 1)
             body += pre
             body.append("#pragma unroll")
-            body.append("for (int cd = 0; cd < NCD; ++cd) out[cd] = %s;" % " + ".join(sum_terms))
+            body.append("for (int cq = 0; cq < NCD; cq += %d) {" % cb)
+            body.append("    R w[%d][%d];" % (nt, cb))
+            body.append("#pragma unroll")
+            body.append("    for (int j = 0; j < %d; ++j) {" % cb)
+            body.append("        if (cq + j < NCD) {")
+            for t, (ei, mono, c) in enumerate(terms):
+                body.append("            w[%d][j] = ws.ld(%d * NCD + cq + j);" % (t, ei))
+            body.append("        }")
+            body.append("    }")
+            body.append("    __builtin_amdgcn_sched_barrier(0);")
+            body.append("#pragma unroll")
+            body.append("    for (int j = 0; j < %d; ++j)" % cb)
+            body.append("        if (cq + j < NCD) out[cq + j] = %s;" % " + ".join(sum_terms))
+            body.append("}")
             cases.append("        case %d: {\n%s\n            break;\n        }" % (
                 ia * nB + ib, "\n".join("            " + l for l in body)))
     code.append("    // out[cd] = (a b | c d) for bra components (ia, ib), iab = ia * NB + ib, from the")
@@ -661,6 +690,7 @@ def gen_class(la, lb, lc, ld):
         flops_exec = (fve, flops_hrr - sum_bra_flops[0] + fbra)
     else:
         code.append("    static constexpr bool STAGED = false;")
+        code.append("    static constexpr int SPLIT = 1;")
     code.append("    static __device__ __forceinline__ void vrr(const R* __restrict__ F, R* __restrict__ acc,")
     code.append("        R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz,")
     code.append("        R WQx, R WQy, R WQz, R inv2p, R inv2q, R inv2pq, R rho_p, R rho_q) {")

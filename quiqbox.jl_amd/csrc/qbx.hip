@@ -153,6 +153,7 @@ struct qbx_ctx {
     bool ket_buckets = true;       // kets bucketed by primitive-pair count (QBX_KET_BUCKETS=0: off)
     bool jk_rows = true;           // LDS K-row and twin kernels in the J/K build (QBX_JK_ROWS=0: off, the
                                    // one-wave-per-chunk kernels run everywhere)
+    bool staged_split = true;      // (dd|dd): each wave task shared by several waves (QBX_STAGED_SPLIT=0: off)
     DevBuf<double> la_X, la_F, la_P, la_Q, la_T, la_aux;   // density purification
     double* scratch = nullptr;     // this device's address of the fixed work array
     rocblas_handle blas = nullptr; // rocSOLVER's handle (qbx_fock_solve), created on first use
@@ -1515,6 +1516,7 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
             a.twmax = w ? w->twmax : 0;
         };
         a.lds_max = ctx->lds_max;
+        a.split = ctx->staged_split ? 1 : 0;
         const TaskSet& t64 = twin ? b->s64[cls] : b->t64[cls];
         set_tasks(t64, twin ? &b->w64[cls] : nullptr);
         attach_workspace(b, cls, std::max(t64.nwave, mode == QBX_PREC_MIXED ? b->t32[cls].nwave : 0), a);
@@ -1707,6 +1709,8 @@ int qbx_init(int32_t device, qbx_ctx** out)
             c->ket_buckets = !(benv && benv[0] == '0');
             const char* renv = std::getenv("QBX_JK_ROWS");
             c->jk_rows = !(renv && renv[0] == '0');
+            const char* senv = std::getenv("QBX_STAGED_SPLIT");
+            c->staged_split = !(senv && senv[0] == '0');
         }
         QBX_HIP(hipStreamSynchronize(c->stream));
         *out = c.release();

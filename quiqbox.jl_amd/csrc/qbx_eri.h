@@ -888,21 +888,26 @@ __device__ __forceinline__ void staged_group(BraPtr BP, int bn, const PrimPair* 
     Cls::template hrr_ket_grp<G>(acc, ws, CDx, CDy, CDz);
 }
 
-template <class Cls, int LT, typename R, typename BraPtr, int... Gs>
-__device__ __forceinline__ void staged_groups(std::integer_sequence<int, Gs...>, BraPtr BP, int bn,
+// The groups of one wave task; with W > 1 waves sharing the task, wave wib
+// (wave-uniform) evaluates groups wib, wib + W, ...
+template <class Cls, int LT, typename R, int W, typename BraPtr, int... Gs>
+__device__ __forceinline__ void staged_groups(std::integer_sequence<int, Gs...>, int wib, BraPtr BP, int bn,
                                               const PrimPair* __restrict__ KP, int kn,
                                               const double* __restrict__ sb, R CDx, R CDy, R CDz,
                                               const WsSlot<R>& ws)
 {
-    (staged_group<Cls, LT, R, Gs>(BP, bn, KP, kn, sb, CDx, CDy, CDz, ws), ...);
+    ((W == 1 || Gs % W == wib ? staged_group<Cls, LT, R, Gs>(BP, bn, KP, kn, sb, CDx, CDy, CDz, ws) : (void)0), ...);
 }
 
 // One wave task of a staged class: bra `bi`, kets chunk*64 .. chunk*64+63 of
-// its list.  Evaluates the groups into the wave's workspace slot, then writes
-// the dense tensor or digests J/K (J through the sink, K through `kout`).
-template <int LA, int LB, int LC, int LD, typename R, int MODE, class KOut>
+// its list.  Evaluates the groups into the workspace slot, then writes the
+// dense tensor or digests J/K (J through the sink, K through `kout`).  With
+// W > 1 the W waves of a block share the task and its slot (wave wib takes
+// every W-th group, then one W-th of the (a b| rows of the digest); every
+// wave of the block must call with the same task.
+template <int LA, int LB, int LC, int LD, typename R, int MODE, class KOut, int W = 1>
 __device__ __forceinline__ void staged_chunk(const EriArgs& a, int tb, int chunk, int lane, const WsSlot<R>& ws,
-                                             const double* sb, const KOut& kout)
+                                             const double* sb, const KOut& kout, int wib = 0)
 {
     using Cls = EriClass<LA, LB, LC, LD, R>;
     constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB), NC = qbx_ncart(LC), ND = qbx_ncart(LD);
@@ -929,9 +934,13 @@ __device__ __forceinline__ void staged_chunk(const EriArgs& a, int tb, int chunk
     const bool live = kn > 0;
 
     const double* CDp = a.ket.AB + 3 * ki;
-    staged_groups<Cls, LT, R>(std::make_integer_sequence<int, Cls::NGRP>{},
-                              cst(a.pp + cst(a.bra.ppo)[bi]), cst(a.bra.ppn)[bi],
-                              a.pp + a.ket.ppo[ki], kn, sb, (R)CDp[0], (R)CDp[1], (R)CDp[2], ws);
+    staged_groups<Cls, LT, R, W>(std::make_integer_sequence<int, Cls::NGRP>{}, wib,
+                                 cst(a.pp + cst(a.bra.ppo)[bi]), cst(a.bra.ppn)[bi],
+                                 a.pp + a.ket.ppo[ki], kn, sb, (R)CDp[0], (R)CDp[1], (R)CDp[2], ws);
+    if constexpr (W > 1) {
+        static_assert(MODE == MODE_JK, "split staged tasks digest J/K only");
+        __syncthreads();    // every group's [e|cd) is in the slot
+    }
 
     const cptr<double> ABp = cst(a.bra.AB + 3 * bi);
     const R ABx = (R)ABp[0], ABy = (R)ABp[1], ABz = (R)ABp[2];
@@ -977,7 +986,7 @@ __device__ __forceinline__ void staged_chunk(const EriArgs& a, int tb, int chunk
         return;
     }
 
-    // ---- J/K digest from the (a b| rows, in two passes that each keep
+    // ---- J/K digest from the (a b| rows of [r0, r1), in two passes that each keep
     //      few sums live: A) J_ab, K_ac, K_ad per row; B) J_cd summed over
     //      all rows, K_bd and K_bc summed over a.  The density elements a
     //      lane reads are loaded once per quartet (its D_cd block), per B
@@ -992,6 +1001,9 @@ __device__ __forceinline__ void staged_chunk(const EriArgs& a, int tb, int chunk
     for (int n = 0; n < NC; ++n) fc[n] = fc[n] < 0 ? 0 : fc[n];
 #pragma unroll
     for (int n = 0; n < ND; ++n) fd[n] = fd[n] < 0 ? 0 : fd[n];
+    // rows t = ib * NA + ia in [r0, r1): all of them, or this wave's share
+    constexpr int NR = NA * NB;
+    const int r0 = W == 1 ? 0 : wib * NR / W, r1 = W == 1 ? NR : (wib + 1) * NR / W;
     const JKSink sink(a);
     for (int s = 0; s < a.nset; ++s) {
         const double* __restrict__ DJ = a.D + (size_t)s * N * N;
@@ -1002,86 +1014,89 @@ __device__ __forceinline__ void staged_chunk(const EriArgs& a, int tb, int chunk
         for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
             for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = DJ[fc[ic] * N + fd[id]];
+        double dbc[NC], dbd[ND];
 #pragma unroll 1
-        for (int ib = 0; ib < NB; ++ib) {
-            const int fb_ = max(SB->bf[ib], 0);
-            double dbc[NC], dbd[ND];
+        for (int t = r0; t < r1; ++t) {
+            const int ib = t / NA, ia = t - ib * NA;
+            if (t == r0 || ia == 0) {
+                const int fb_ = max(SB->bf[ib], 0);
 #pragma unroll
-            for (int ic = 0; ic < NC; ++ic) dbc[ic] = DK[fb_ * N + fc[ic]];
+                for (int ic = 0; ic < NC; ++ic) dbc[ic] = DK[fb_ * N + fc[ic]];
 #pragma unroll
-            for (int id = 0; id < ND; ++id) dbd[id] = DK[fb_ * N + fd[id]];
-#pragma unroll 1
-            for (int ia = 0; ia < NA; ++ia) {
-                R out[NCD];
-                Cls::hrr_bra_ab(ia * NB + ib, ws, out, ABx, ABy, ABz);
-                const int fa_ = max(SA->bf[ia], 0);
-                const double sab = SA->sc[ia] * SB->sc[ib];
-                double jab = 0.0, kac[NC], kad[ND];
+                for (int id = 0; id < ND; ++id) dbd[id] = DK[fb_ * N + fd[id]];
+            }
+            R out[NCD];
+            Cls::hrr_bra_ab(ia * NB + ib, ws, out, ABx, ABy, ABz);
+            const int fa_ = max(SA->bf[ia], 0), fb_ = max(SB->bf[ib], 0);
+            const double sab = SA->sc[ia] * SB->sc[ib];
+            double jab = 0.0, kac[NC], kad[ND];
 #pragma unroll
-                for (int n = 0; n < NC; ++n) kac[n] = 0.0;
+            for (int n = 0; n < NC; ++n) kac[n] = 0.0;
 #pragma unroll
-                for (int n = 0; n < ND; ++n) kad[n] = 0.0;
+            for (int n = 0; n < ND; ++n) kad[n] = 0.0;
 #pragma unroll
-                for (int ic = 0; ic < NC; ++ic)
+            for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
-                    for (int id = 0; id < ND; ++id) {
-                        const double v = (double)out[ic * ND + id] * sab * (scc[ic] * scdd[id]);
-                        jab += v * dcd[ic * ND + id];
-                        kac[ic] += v * dbd[id];
-                        kad[id] += v * dbc[ic];
-                    }
-                jab = wave_sum(fj * jab);
-                if (lane == 0) sink.add(Js + fa_ * N + fb_, jab);
-                if (live) {
-#pragma unroll
-                    for (int ic = 0; ic < NC; ++ic) kout.add(s, fa_, ia, fc[ic], fk * kac[ic]);
-#pragma unroll
-                    for (int id = 0; id < ND; ++id) kout.add(s, fa_, ia, fd[id], fk * kad[id]);
+                for (int id = 0; id < ND; ++id) {
+                    const double v = (double)out[ic * ND + id] * sab * (scc[ic] * scdd[id]);
+                    jab += v * dcd[ic * ND + id];
+                    kac[ic] += v * dbd[id];
+                    kad[id] += v * dbc[ic];
                 }
-            }
-        }
-        double jcd[NCD];
-#pragma unroll
-        for (int n = 0; n < NCD; ++n) jcd[n] = 0.0;
-#pragma unroll 1
-        for (int ib = 0; ib < NB; ++ib) {
-            const int fb_ = max(SB->bf[ib], 0);
-            double kbd[ND], kbc[NC];
-#pragma unroll
-            for (int n = 0; n < ND; ++n) kbd[n] = 0.0;
-#pragma unroll
-            for (int n = 0; n < NC; ++n) kbc[n] = 0.0;
-#pragma unroll 1
-            for (int ia = 0; ia < NA; ++ia) {
-                R out[NCD];
-                Cls::hrr_bra_ab(ia * NB + ib, ws, out, ABx, ABy, ABz);
-                const int fa_ = max(SA->bf[ia], 0);
-                const double sab = SA->sc[ia] * SB->sc[ib];
-                const double dab = cst(DJ)[fa_ * N + fb_];
-                double dac[NC], dad[ND];
-#pragma unroll
-                for (int ic = 0; ic < NC; ++ic) dac[ic] = DK[fa_ * N + fc[ic]];
-#pragma unroll
-                for (int id = 0; id < ND; ++id) dad[id] = DK[fa_ * N + fd[id]];
-#pragma unroll
-                for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-                    for (int id = 0; id < ND; ++id) {
-                        const int cd = ic * ND + id;
-                        const double v = (double)out[cd] * sab * (scc[ic] * scdd[id]);
-                        jcd[cd] += v * dab;
-                        kbd[id] += v * dac[ic];
-                        kbc[ic] += v * dad[id];
-                    }
-            }
+            jab = wave_sum(fj * jab);
+            if (lane == 0) sink.add(Js + fa_ * N + fb_, jab);
             if (live) {
 #pragma unroll
-                for (int id = 0; id < ND; ++id) kout.add(s, fb_, NA + ib, fd[id], fk * kbd[id]);
+                for (int ic = 0; ic < NC; ++ic) kout.add(s, fa_, ia, fc[ic], fk * kac[ic]);
 #pragma unroll
-                for (int ic = 0; ic < NC; ++ic) kout.add(s, fb_, NA + ib, fc[ic], fk * kbc[ic]);
+                for (int id = 0; id < ND; ++id) kout.add(s, fa_, ia, fd[id], fk * kad[id]);
             }
         }
-        if (live) {
+        double jcd[NCD], kbd[ND], kbc[NC];
+#pragma unroll
+        for (int n = 0; n < NCD; ++n) jcd[n] = 0.0;
+#pragma unroll
+        for (int n = 0; n < ND; ++n) kbd[n] = 0.0;
+#pragma unroll
+        for (int n = 0; n < NC; ++n) kbc[n] = 0.0;
+#pragma unroll 1
+        for (int t = r0; t < r1; ++t) {
+            const int ib = t / NA, ia = t - ib * NA;
+            R out[NCD];
+            Cls::hrr_bra_ab(ia * NB + ib, ws, out, ABx, ABy, ABz);
+            const int fa_ = max(SA->bf[ia], 0), fb_ = max(SB->bf[ib], 0);
+            const double sab = SA->sc[ia] * SB->sc[ib];
+            const double dab = cst(DJ)[fa_ * N + fb_];
+            double dac[NC], dad[ND];
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) dac[ic] = DK[fa_ * N + fc[ic]];
+#pragma unroll
+            for (int id = 0; id < ND; ++id) dad[id] = DK[fa_ * N + fd[id]];
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                for (int id = 0; id < ND; ++id) {
+                    const int cd = ic * ND + id;
+                    const double v = (double)out[cd] * sab * (scc[ic] * scdd[id]);
+                    jcd[cd] += v * dab;
+                    kbd[id] += v * dac[ic];
+                    kbc[ic] += v * dad[id];
+                }
+            // K_bd, K_bc are complete for this b once its last row of the range is in
+            if (ia == NA - 1 || t == r1 - 1) {
+                if (live) {
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) kout.add(s, fb_, NA + ib, fd[id], fk * kbd[id]);
+#pragma unroll
+                    for (int ic = 0; ic < NC; ++ic) kout.add(s, fb_, NA + ib, fc[ic], fk * kbc[ic]);
+                }
+#pragma unroll
+                for (int n = 0; n < ND; ++n) kbd[n] = 0.0;
+#pragma unroll
+                for (int n = 0; n < NC; ++n) kbc[n] = 0.0;
+            }
+        }
+        if (live && r1 > r0) {
 #pragma unroll
             for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
@@ -1118,6 +1133,41 @@ void eri_staged_kernel(EriArgs a)
             if (cpre[mid] <= wave) lo = mid; else hi = mid;
         }
         staged_chunk<LA, LB, LC, LD, R, MODE>(a, lo, wave - cpre[lo], lane, ws, sb, kout);
+    }
+}
+
+// Staged J/K with each wave task shared by the Cls::SPLIT waves of a block
+// (classes with few tasks: their latency, not the throughput of a full chip,
+// sets the time).  A persistent grid of blocks, one workspace slot each.
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ __launch_bounds__((64 * EriClass<LA, LB, LC, LD, R>::SPLIT)) __attribute__((amdgpu_waves_per_eu(EriClass<LA, LB, LC, LD, R>::WAVES_PER_SIMD, EriClass<LA, LB, LC, LD, R>::WAVES_PER_SIMD)))
+void eri_staged_split_kernel(EriArgs a)
+{
+    using Cls = EriClass<LA, LB, LC, LD, R>;
+    constexpr int W = Cls::SPLIT;
+    constexpr int TPB = 64 * W;
+    constexpr int NCD = qbx_ncart(LC) * qbx_ncart(LD);
+    constexpr int WSV = Cls::NE * NCD;
+    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
+    stage_boys<TPB>(sboys, a.boysl);
+    __syncthreads();
+    const double* sb = (const double*)sboys;
+
+    const int lane = (int)threadIdx.x & 63;
+    const int wib = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const WsSlot<R> ws((R*)a.ws + (size_t)blockIdx.x * WSV * 64, WSV, lane);
+    const cptr<int> cpre = cst(a.cpre);
+    const JKSink sink(a);
+    const KToGlobal kout{sink, (size_t)a.N};
+
+    for (int task = (int)blockIdx.x; task < a.nwave; task += (int)gridDim.x) {
+        int lo = 0, hi = a.nbra;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (cpre[mid] <= task) lo = mid; else hi = mid;
+        }
+        staged_chunk<LA, LB, LC, LD, R, MODE_JK, KToGlobal, W>(a, lo, task - cpre[lo], lane, ws, sb, kout, wib);
+        __syncthreads();    // the slot is rewritten by the next task
     }
 }
 
@@ -1250,6 +1300,16 @@ void launch_eri(const EriArgs& a, hipStream_t s)
                 if (blocks <= 0) throw QbxError(QBX_EINTERNAL, "staged ERI workspace holds no block");
                 hipLaunchKernelGGL((eri_staged_rows_kernel<LA, LB, LC, LD, R>), dim3((unsigned)blocks),
                                    dim3(64 * STAGE_WAVES_PER_BLOCK), shm, s, a);
+                QBX_HIP(hipGetLastError());
+                return;
+            }
+        }
+        if constexpr (MODE == MODE_JK && EriClass<LA, LB, LC, LD, R>::SPLIT > 1) {
+            if (a.split) {
+                // one slot per block
+                const int blocks = a.nwave < a.ws_slots ? a.nwave : a.ws_slots;
+                hipLaunchKernelGGL((eri_staged_split_kernel<LA, LB, LC, LD, R>), dim3((unsigned)blocks),
+                                   dim3(64 * EriClass<LA, LB, LC, LD, R>::SPLIT), 0, s, a);
                 QBX_HIP(hipGetLastError());
                 return;
             }

@@ -88,6 +88,7 @@ struct EriArgs {
     // EriClass::NE * NCD * 64 values of the kernel's arithmetic type
     void* ws;
     int ws_slots;
+    int split;                // classes with Cls::SPLIT > 1: share each wave task among SPLIT waves (0: off)
     // gradient kernels: radial indices (k_a | k_b << 16) of every primitive pair,
     // first radial primitive of every shell, and the outputs: per radial
     // primitive dE/d(exponent) and c dE/dc, per shell dE/d(centre)
