@@ -79,13 +79,15 @@ def ncart_prod(c):
 # instead of 31.  Measured on the 592-function cluster: (dd|dd) 4.1 -> 2.7 ms;
 # the other staged classes lose more latency hiding than they save.
 STAGE_ONE_WAVE = {(2, 2, 2, 2): 48}
-# Staged classes whose groups of one wave task are spread over the waves of
-# a block (value: waves per task).  (dd|dd) has too few tasks (12470 shell
-# quartets on the 592-function cluster) to give every SIMD more than one
-# wave; four waves per task measured 2.78 -> 1.25 ms.  (dd|dp) at eight waves
-# per task: 4.45 -> 4.38 ms alone, but its whole-CU blocks waited behind the
-# other classes of the concurrent build (7.7 -> 25.7 ms there).
-STAGE_SPLIT = {(2, 2, 2, 2): 4}
+# Staged classes whose 64-ket chunks are each shared by the waves of a block
+# (value: waves per block; groups, then digest rows, spread over them; the
+# bra's K rows in LDS).  Measured serialized on the 592-function cluster,
+# against one wave per chunk: (dd|dd) 2.78 -> 0.82 ms (12470 shell quartets,
+# too few tasks to give every SIMD more than one wave), (dd|dp) 3.89 -> 2.66
+# ms, (dp|dp) 2.50 -> 2.14 ms; (dp|pp) 3.72 -> 4.29, (dd|pp) 1.47 -> 1.61 and
+# (dd|ds) 1.13 -> 1.27 ms lost (many tasks already fill the chip, and the
+# extra barriers cost).
+STAGE_SPLIT = {(2, 2, 2, 2): 4, (2, 2, 2, 1): 4, (2, 1, 2, 1): 4}
 TWIN_MAX_ACC = 9   # must match qbx_eri_api.h
 
 

@@ -153,7 +153,7 @@ struct qbx_ctx {
     bool ket_buckets = true;       // kets bucketed by primitive-pair count (QBX_KET_BUCKETS=0: off)
     bool jk_rows = true;           // LDS K-row and twin kernels in the J/K build (QBX_JK_ROWS=0: off, the
                                    // one-wave-per-chunk kernels run everywhere)
-    bool staged_split = true;      // (dd|dd): each wave task shared by several waves (QBX_STAGED_SPLIT=0: off)
+    bool staged_split = true;      // (dd|dd), (dd|dp), (dp|dp): 64-ket chunks shared by a block's waves (QBX_STAGED_SPLIT=0: off)
     DevBuf<double> la_X, la_F, la_P, la_Q, la_T, la_aux;   // density purification
     double* scratch = nullptr;     // this device's address of the fixed work array
     rocblas_handle blas = nullptr; // rocSOLVER's handle (qbx_fock_solve), created on first use

@@ -1224,6 +1224,60 @@ void eri_staged_rows_kernel(EriArgs a)
     }
 }
 
+// Staged J/K with the bra's K rows in LDS and each 64-ket chunk shared by
+// the Cls::SPLIT waves of a block (groups, then digest rows, spread over the
+// waves; one workspace slot per block).  A task is one (bra, up to
+// STAGED_ROW_CHUNKS chunks) range, its chunks taken one after another by the
+// whole block, so no wave idles when a bra has fewer chunks than the block
+// has waves; the rows are flushed to K once per task.
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ __launch_bounds__((64 * EriClass<LA, LB, LC, LD, R>::SPLIT)) __attribute__((amdgpu_waves_per_eu(EriClass<LA, LB, LC, LD, R>::WAVES_PER_SIMD, EriClass<LA, LB, LC, LD, R>::WAVES_PER_SIMD)))
+void eri_staged_split_rows_kernel(EriArgs a)
+{
+    using Cls = EriClass<LA, LB, LC, LD, R>;
+    constexpr int W = Cls::SPLIT;
+    constexpr int TPB = 64 * W;
+    constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB);
+    constexpr int NROW = NA + NB;
+    constexpr int NCD = qbx_ncart(LC) * qbx_ncart(LD);
+    constexpr int WSV = Cls::NE * NCD;
+    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
+    extern __shared__ double krow[];    // [nset][NROW][N]
+    const int Ni = a.N;
+    const int nrowel = a.nset * NROW * Ni;
+    stage_boys<TPB>(sboys, a.boysl);
+    for (int i = threadIdx.x; i < nrowel; i += TPB) krow[i] = 0.0;
+    __syncthreads();
+    const double* sb = (const double*)sboys;
+
+    const int lane = (int)threadIdx.x & 63;
+    const int wib = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const WsSlot<R> ws((R*)a.ws + (size_t)blockIdx.x * WSV * 64, WSV, lane);
+    const cptr<int> spre = cst(a.spre);
+    const JKSink sink(a);
+    const KToRows<NROW> kout{krow, Ni};
+
+    for (int task = (int)blockIdx.x; task < a.nsuper; task += (int)gridDim.x) {
+        int lo = 0, hi = a.nbra;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (spre[mid] <= task) lo = mid; else hi = mid;
+        }
+        const int tb = lo, bi = tb / a.nbk;
+        const int sc = task - spre[tb];
+        const int nk = (int)(cst(a.prefix)[tb + 1] - cst(a.prefix)[tb]);
+        for (int r = 0; r < STAGED_ROW_CHUNKS; ++r) {
+            const int chunk = sc * STAGED_ROW_CHUNKS + r;
+            if (chunk * 64 >= nk) break;
+            staged_chunk<LA, LB, LC, LD, R, MODE_JK, KToRows<NROW>, W>(a, tb, chunk, lane, ws, sb, kout, wib);
+            __syncthreads();    // the slot is rewritten by the next chunk
+        }
+        const int sA = cst(a.bra.shA)[bi], sB = cst(a.bra.shB)[bi];
+        flush_rows<NA, TPB>(krow, nrowel, NROW, Ni, a.sh[sA].bf, a.sh[sB].bf, sink, true);
+        __syncthreads();
+    }
+}
+
 // Schwarz bounds: one thread per diagonal shell pair (ab|ab).
 template <int LA, int LB, typename R>
 __global__ __launch_bounds__(256) void eri_schwarz_kernel(EriArgs a)
@@ -1279,8 +1333,10 @@ void launch_eri(const EriArgs& a, hipStream_t s)
         if (!a.ws) throw QbxError(QBX_EINTERNAL, "staged ERI class launched without a workspace");
         // LDS K rows pay where K updates are dense: (NA+NB)(NC+ND) updates
         // per quartet against NA NB NC ND integrals, at least 1 to 5.  The
-        // sparser (dd|dp) and (dd|dd) lose more to the block's shared task
-        // (waves idle at its end, fewer tasks in flight) than they save.
+        // sparser (dd|dp) and (dd|dd) get them with chunks shared by the
+        // block's waves (SPLIT > 1); with one wave per chunk they lose more
+        // to the block's shared task (waves idle at its end, fewer tasks in
+        // flight) than they save.
         constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB), NC = qbx_ncart(LC), ND = qbx_ncart(LD);
         constexpr bool DENSE_K = 5 * (NA + NB) * (NC + ND) >= NA * NB * NC * ND;
         if constexpr (MODE == MODE_JK && DENSE_K) {
@@ -1293,7 +1349,8 @@ void launch_eri(const EriArgs& a, hipStream_t s)
                 (void)hipGetLastError();
                 return ok;
             }();
-            if (rows_ok && a.spre && shm <= (size_t)ROW_LDS_MAX && shm + boys_lds <= (size_t)a.lds_max) {
+            if ((EriClass<LA, LB, LC, LD, R>::SPLIT == 1 || !a.split) && rows_ok && a.spre &&
+                shm <= (size_t)ROW_LDS_MAX && shm + boys_lds <= (size_t)a.lds_max) {
                 if (a.nsuper <= 0) return;
                 const int max_blocks = a.ws_slots / STAGE_WAVES_PER_BLOCK;
                 const int blocks = a.nsuper < max_blocks ? a.nsuper : max_blocks;
@@ -1305,6 +1362,25 @@ void launch_eri(const EriArgs& a, hipStream_t s)
             }
         }
         if constexpr (MODE == MODE_JK && EriClass<LA, LB, LC, LD, R>::SPLIT > 1) {
+            constexpr int W = EriClass<LA, LB, LC, LD, R>::SPLIT;
+            const size_t shm = (size_t)a.nset * (NA + NB) * a.N * sizeof(double);
+            const size_t boys_lds = QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE * sizeof(double);
+            static const bool srows_ok = [] {
+                const bool ok = hipFuncSetAttribute((const void*)eri_staged_split_rows_kernel<LA, LB, LC, LD, R>,
+                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                    ROW_LDS_MAX) == hipSuccess;
+                (void)hipGetLastError();
+                return ok;
+            }();
+            if (a.split && srows_ok && a.spre && shm <= (size_t)ROW_LDS_MAX && shm + boys_lds <= (size_t)a.lds_max) {
+                if (a.nsuper <= 0) return;
+                // one slot per block
+                const int blocks = a.nsuper < a.ws_slots ? a.nsuper : a.ws_slots;
+                hipLaunchKernelGGL((eri_staged_split_rows_kernel<LA, LB, LC, LD, R>), dim3((unsigned)blocks),
+                                   dim3(64 * W), shm, s, a);
+                QBX_HIP(hipGetLastError());
+                return;
+            }
             if (a.split) {
                 // one slot per block
                 const int blocks = a.nwave < a.ws_slots ? a.nwave : a.ws_slots;

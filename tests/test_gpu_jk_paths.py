@@ -8,6 +8,10 @@ ordering of builds across streams.
   dimer in cc-pVDZ (ket buckets on) and on a small s/p/d cluster whose staged
   classes then take their per-wave kernel: one and two density sets, FP64 and
   mixed precision.
+- QBX_STAGED_SPLIT=0 (fresh context) gives (dd|dd), (dd|dp) and (dp|dp)
+  back their one-wave-per-chunk staged kernels (by default the waves of a
+  block share each chunk); with QBX_JK_ROWS=0 those classes take the shared
+  chunks with K straight to global memory.  Both against the oracle.
 - A build enqueued on a caller's stream (qbx_jk_device), followed by a build
   with another Schwarz threshold and precision on the context stream (which
   rewrites the task lists): both against the oracle.
@@ -78,6 +82,25 @@ def test_one_wave_per_chunk_jk_against_oracle(system):
     gd = qb.GPUBasis(bs)
     Gd = gd.jk(DJ, DK, 0.0)
     assert max(float(np.max(np.abs(Gd[s] - G2[s]))) for s in range(2)) < 1e-12 * max(1.0, gmax)
+    del gb
+    ctx.close()
+
+
+@pytest.mark.parametrize("env", [{"QBX_STAGED_SPLIT": "0"}, {"QBX_STAGED_SPLIT": "0", "QBX_JK_ROWS": "0"}])
+def test_staged_kernels_without_shared_chunks(env):
+    _, bs = random_spd_basis(31, n_atoms=4)
+    ctx = _context(**env)
+    gb = qb.GPUBasis(bs, ctx=ctx)
+    E = OracleEngine(gb.flat).eri()
+    n = gb.n
+    rng = np.random.default_rng(5)
+    DJ = np.stack([_sym(rng, n), _sym(rng, n)])
+    DK = np.stack([_sym(rng, n), _sym(rng, n)])
+    Gref = [oracle.gcore(E, DJ[s], DK[s]) for s in range(2)]
+    G = gb.jk(DJ, DK, 0.0)
+    err = max(float(np.max(np.abs(G[s] - Gref[s]))) for s in range(2))
+    _say("spd cluster (%d functions), %s: max|G - G_oracle| %.3e" % (n, env, err))
+    assert err < 1e-11
     del gb
     ctx.close()
 

@@ -156,7 +156,13 @@ def test_fock_solve_matches_host_eigen_step(system):
     err_D = float(np.max(np.abs(D - D_ref)))
     gaps = np.minimum(np.diff(e_ref, prepend=-np.inf), np.diff(e_ref, append=np.inf))
     nondeg = gaps > 1e-6
-    err_C = float(np.max(np.abs(C[:, nondeg] - C_ref[:, nondeg])))
+    # the sign rule (C[0, j] >= 0, as the reference's) leaves the sign of a
+    # column whose first coefficient vanishes by symmetry to rounding: such
+    # columns (H2O2's C2 axis gives some) are compared up to sign
+    dC = np.abs(C - C_ref)
+    free = np.abs(C_ref[0, :]) < 1e-8
+    dC[:, free] = np.minimum(dC[:, free], np.abs(C + C_ref)[:, free])
+    err_C = float(np.max(dC[:, nondeg]))
     _say("%s (n %d): max|de| %.2e, max|dD| %.2e, max|dC| (nondegenerate) %.2e" % (system, gb.n, err_e, err_D, err_C))
     assert np.all(C[0, :] >= 0)
     assert err_e < 1e-11 * max(1.0, float(np.max(np.abs(e_ref))))
