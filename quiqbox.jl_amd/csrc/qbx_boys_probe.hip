@@ -10,13 +10,13 @@ template <int L>
 __global__ __launch_bounds__(256) void boys_probe_kernel(const double* __restrict__ rows, const double* __restrict__ T,
                                                          int n, double* __restrict__ F)
 {
-    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
+    __shared__ double2 sboys[BOYS_HEAD2 + QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
     stage_boys<256>(sboys, rows);
     __syncthreads();
     const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
     if (i >= n) return;
     double f[L + 1];
-    boys_lds<L>(T[i], f, (const double*)sboys);
+    boys_lds<L>(T[i], f, (const double*)sboys + BOYS_HEAD);
 #pragma unroll
     for (int m = 0; m <= L; ++m) F[(size_t)i * (L + 1) + m] = f[m];
 }

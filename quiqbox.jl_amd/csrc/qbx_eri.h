@@ -73,6 +73,16 @@ constexpr int boys_rows(int L) { return (int)(kBoysTmax[L] * QBX_BOYSL_INVDT) + 
 static_assert(kBoysTmax[QBX_BOYSL_LMAX] <= QBX_BOYS_TMAX && boys_rows(QBX_BOYSL_LMAX) <= QBX_BOYSL_ROWS,
               "the per-order extent must stay inside the table");
 
+// LDS copy of the Boys table: BOYS_HEAD doubles of constants (1/k! for
+// k = 11..0, the Taylor coefficients of e^x), then the rows; kernels hold
+// sb = table + BOYS_HEAD.  The constants come from LDS rather than from the
+// code: as literals the compiler kept them in VGPRs and copied one into the
+// accumulator before each Horner step of e^x (v_mov_b64 + v_fmac_f64, 22
+// VALU instructions per primitive quartet instead of 11), while a loaded value
+// is consumed in place.
+constexpr int BOYS_HEAD = 12;
+constexpr int BOYS_HEAD2 = BOYS_HEAD / 2;    // the head in double2
+
 // Boys F_0..F_L from the LDS rows of order L (see qbx_device.h for the table):
 // an 11-term Taylor expansion of F_L about the nearest grid point, e^{-T} from
 // e^{-T_i} times the Taylor series of e^{T_i - T} (|T_i - T| <= 1/8, terms to
@@ -98,17 +108,19 @@ __device__ __forceinline__ void boys_lds(double T, double* __restrict__ F, const
         acc = fma(md, acc, c01.x);
         F[L] = acc;
         if (L > 0) {
-            double ex = fma(md, 1.0 / 39916800.0, 1.0 / 3628800.0);
-            ex = fma(md, ex, 1.0 / 362880.0);
-            ex = fma(md, ex, 1.0 / 40320.0);
-            ex = fma(md, ex, 1.0 / 5040.0);
-            ex = fma(md, ex, 1.0 / 720.0);
-            ex = fma(md, ex, 1.0 / 120.0);
-            ex = fma(md, ex, 1.0 / 24.0);
-            ex = fma(md, ex, 1.0 / 6.0);
-            ex = fma(md, ex, 0.5);
-            ex = fma(md, ex, 1.0);
-            ex = fma(md, ex, 1.0);
+            const double2* kh = (const double2*)(sb - BOYS_HEAD);
+            const double2 k01 = kh[0], k23 = kh[1], k45 = kh[2], k67 = kh[3], k89 = kh[4], kab = kh[5];
+            double ex = fma(md, k01.x, k01.y);
+            ex = fma(md, ex, k23.x);
+            ex = fma(md, ex, k23.y);
+            ex = fma(md, ex, k45.x);
+            ex = fma(md, ex, k45.y);
+            ex = fma(md, ex, k67.x);
+            ex = fma(md, ex, k67.y);
+            ex = fma(md, ex, k89.x);
+            ex = fma(md, ex, k89.y);
+            ex = fma(md, ex, kab.x);
+            ex = fma(md, ex, kab.y);
             const double e = c10e.y * ex;
             const double t2 = 2.0 * T;
 #pragma unroll
@@ -123,12 +135,20 @@ __device__ __forceinline__ void boys_lds(double T, double* __restrict__ F, const
     }
 }
 
-// Copy the Boys rows of the class's total angular momentum into LDS.
+// Copy the head constants and the Boys rows of the class's total angular
+// momentum into LDS (dst holds BOYS_HEAD2 + ROWS * QBX_BOYSL_STRIDE / 2 double2).
 template <int TPB, int ROWS = QBX_BOYSL_ROWS>
 __device__ __forceinline__ void stage_boys(double2* __restrict__ dst, const double* __restrict__ src)
 {
+    if (threadIdx.x < BOYS_HEAD2) {
+        // 1/k! for k = 11 - 2t and 10 - 2t
+        const int k = 11 - 2 * (int)threadIdx.x;
+        double f = 1.0;
+        for (int j = 2; j < k; ++j) f *= j;
+        dst[threadIdx.x] = make_double2(1.0 / (f * k), 1.0 / f);
+    }
     const double2* s = (const double2*)src;
-    for (int i = threadIdx.x; i < ROWS * QBX_BOYSL_STRIDE / 2; i += TPB) dst[i] = s[i];
+    for (int i = threadIdx.x; i < ROWS * QBX_BOYSL_STRIDE / 2; i += TPB) dst[BOYS_HEAD2 + i] = s[i];
 }
 
 // 1/sqrt(x) for finite x > 0 (p + q of a primitive quartet): the hardware
@@ -232,10 +252,10 @@ __global__ __launch_bounds__(eri_tpb(EriClass<LA, LB, LC, LD, R>::NACC)) void er
     constexpr int LT = LA + LB + LC + LD;
     constexpr int NCD = NC * ND;
     constexpr bool PREFETCH = Cls::NACC <= 18;
-    __shared__ double2 sboys[boys_rows(LT) * QBX_BOYSL_STRIDE / 2];
+    __shared__ double2 sboys[BOYS_HEAD2 + boys_rows(LT) * QBX_BOYSL_STRIDE / 2];
     stage_boys<TPB, boys_rows(LT)>(sboys, a.boysl);
     __syncthreads();
-    const double* sb = (const double*)sboys;
+    const double* sb = (const double*)sboys + BOYS_HEAD;
 
     const int lane = (int)threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)blockIdx.x * (TPB / 64) + ((int)threadIdx.x >> 6));
@@ -594,7 +614,7 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void eri_jk_rows_kernel(EriArgs a)
     constexpr int NCD = NC * ND;
     constexpr int NROW = NA + NB;
     constexpr bool PREFETCH = Cls::NACC <= 18;
-    __shared__ double2 sboys[boys_rows(LT) * QBX_BOYSL_STRIDE / 2];
+    __shared__ double2 sboys[BOYS_HEAD2 + boys_rows(LT) * QBX_BOYSL_STRIDE / 2];
     extern __shared__ double krow[];    // [nset][NROW][N]
     const int Ni = a.N;
     const size_t N = (size_t)a.N;
@@ -602,7 +622,7 @@ __global__ __launch_bounds__(64 * ROW_WAVES) void eri_jk_rows_kernel(EriArgs a)
     stage_boys<TPB, boys_rows(LT)>(sboys, a.boysl);
     for (int i = threadIdx.x; i < nrowel; i += TPB) krow[i] = 0.0;
     __syncthreads();
-    const double* sb = (const double*)sboys;
+    const double* sb = (const double*)sboys + BOYS_HEAD;
 
     const int lane = (int)threadIdx.x & 63;
     const int wib = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -690,7 +710,7 @@ __global__ __launch_bounds__(64 * TWIN_WAVES) void eri_jk_twin_kernel(EriArgs a)
     constexpr int NROW = NA + NB;
     constexpr int TW = TWIN_MAX;
     static_assert(ROW_CHUNKS % TWIN_WAVES == 0, "chunks per task must split evenly over the waves");
-    __shared__ double2 sboys[boys_rows(LT) * QBX_BOYSL_STRIDE / 2];
+    __shared__ double2 sboys[BOYS_HEAD2 + boys_rows(LT) * QBX_BOYSL_STRIDE / 2];
     extern __shared__ double krow[];    // [nset][twmax][NROW][N]
     const int Ni = a.N;
     const int tw = a.twmax;
@@ -698,7 +718,7 @@ __global__ __launch_bounds__(64 * TWIN_WAVES) void eri_jk_twin_kernel(EriArgs a)
     stage_boys<TPB, boys_rows(LT)>(sboys, a.boysl);
     for (int i = threadIdx.x; i < nrowel; i += TPB) krow[i] = 0.0;
     __syncthreads();
-    const double* sb = (const double*)sboys;
+    const double* sb = (const double*)sboys + BOYS_HEAD;
 
     const int lane = (int)threadIdx.x & 63;
     const int wib = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -1113,10 +1133,10 @@ void eri_staged_kernel(EriArgs a)
     constexpr int TPB = 64 * STAGE_WAVES_PER_BLOCK;
     constexpr int NCD = qbx_ncart(LC) * qbx_ncart(LD);
     constexpr int WSV = Cls::NE * NCD;          // workspace values per lane
-    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
+    __shared__ double2 sboys[BOYS_HEAD2 + QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
     stage_boys<TPB>(sboys, a.boysl);
     __syncthreads();
-    const double* sb = (const double*)sboys;
+    const double* sb = (const double*)sboys + BOYS_HEAD;
 
     const int lane = (int)threadIdx.x & 63;
     const int slot = __builtin_amdgcn_readfirstlane((int)blockIdx.x * STAGE_WAVES_PER_BLOCK + ((int)threadIdx.x >> 6));
@@ -1148,10 +1168,10 @@ void eri_staged_split_kernel(EriArgs a)
     constexpr int TPB = 64 * W;
     constexpr int NCD = qbx_ncart(LC) * qbx_ncart(LD);
     constexpr int WSV = Cls::NE * NCD;
-    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
+    __shared__ double2 sboys[BOYS_HEAD2 + QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
     stage_boys<TPB>(sboys, a.boysl);
     __syncthreads();
-    const double* sb = (const double*)sboys;
+    const double* sb = (const double*)sboys + BOYS_HEAD;
 
     const int lane = (int)threadIdx.x & 63;
     const int wib = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -1187,14 +1207,14 @@ void eri_staged_rows_kernel(EriArgs a)
     constexpr int NCD = qbx_ncart(LC) * qbx_ncart(LD);
     constexpr int WSV = Cls::NE * NCD;
     static_assert(STAGED_ROW_CHUNKS % W == 0, "chunks per task must split evenly over the waves");
-    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
+    __shared__ double2 sboys[BOYS_HEAD2 + QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
     extern __shared__ double krow[];    // [nset][NROW][N]
     const int Ni = a.N;
     const int nrowel = a.nset * NROW * Ni;
     stage_boys<TPB>(sboys, a.boysl);
     for (int i = threadIdx.x; i < nrowel; i += TPB) krow[i] = 0.0;
     __syncthreads();
-    const double* sb = (const double*)sboys;
+    const double* sb = (const double*)sboys + BOYS_HEAD;
 
     const int lane = (int)threadIdx.x & 63;
     const int wib = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -1241,14 +1261,14 @@ void eri_staged_split_rows_kernel(EriArgs a)
     constexpr int NROW = NA + NB;
     constexpr int NCD = qbx_ncart(LC) * qbx_ncart(LD);
     constexpr int WSV = Cls::NE * NCD;
-    __shared__ double2 sboys[QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
+    __shared__ double2 sboys[BOYS_HEAD2 + QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
     extern __shared__ double krow[];    // [nset][NROW][N]
     const int Ni = a.N;
     const int nrowel = a.nset * NROW * Ni;
     stage_boys<TPB>(sboys, a.boysl);
     for (int i = threadIdx.x; i < nrowel; i += TPB) krow[i] = 0.0;
     __syncthreads();
-    const double* sb = (const double*)sboys;
+    const double* sb = (const double*)sboys + BOYS_HEAD;
 
     const int lane = (int)threadIdx.x & 63;
     const int wib = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -1286,7 +1306,7 @@ __global__ __launch_bounds__(256) void eri_schwarz_kernel(EriArgs a)
     constexpr int NA = qbx_ncart(LA), NB = qbx_ncart(LB);
     constexpr int LT = 2 * (LA + LB);
     constexpr int NCD = NA * NB;
-    __shared__ double2 sboys[boys_rows(LT) * QBX_BOYSL_STRIDE / 2];
+    __shared__ double2 sboys[BOYS_HEAD2 + boys_rows(LT) * QBX_BOYSL_STRIDE / 2];
     stage_boys<256, boys_rows(LT)>(sboys, a.boysl);
     __syncthreads();
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1297,7 +1317,7 @@ __global__ __launch_bounds__(256) void eri_schwarz_kernel(EriArgs a)
     for (int n = 0; n < Cls::NACC; ++n) acc[n] = (R)0;
     const PrimPair* P = a.pp + a.bra.ppo[bi];
     const int n = a.bra.ppn[bi];
-    prim_loop<Cls, LT, R, false, -1>(acc, P, n, P, n, (const double*)sboys);
+    prim_loop<Cls, LT, R, false, -1>(acc, P, n, P, n, (const double*)sboys + BOYS_HEAD);
     const double* AB = a.bra.AB + 3 * bi;
     R kacc[Cls::NE * NCD];
     Cls::hrr_ket(acc, kacc, (R)AB[0], (R)AB[1], (R)AB[2]);

@@ -119,10 +119,10 @@ __global__ __launch_bounds__(256) void eri_grad_kernel(EriArgs a)
     constexpr int LT = LA + LB + LC + LD;
     constexpr int NACC = Adj::NACC;
     constexpr int NINT = Adj::NINT;
-    __shared__ double2 sboys[boys_rows(LT + 1) * QBX_BOYSL_STRIDE / 2];
+    __shared__ double2 sboys[BOYS_HEAD2 + boys_rows(LT + 1) * QBX_BOYSL_STRIDE / 2];
     stage_boys<256, boys_rows(LT + 1)>(sboys, a.boysl);    // rows of order LT + 1
     __syncthreads();
-    const double* sb = (const double*)sboys;
+    const double* sb = (const double*)sboys + BOYS_HEAD;
 
     const int lane = (int)threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)blockIdx.x * 4 + ((int)threadIdx.x >> 6));
