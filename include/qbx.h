@@ -163,6 +163,25 @@ int  qbx_last_timing(qbx_ctx* ctx, float* ms_per_class, float* ms_total);
    (n x (order+1), row-major).  order in [0, 8]. */
 int  qbx_boys_device(qbx_ctx* ctx, int32_t order, int32_t n, const double* T, double* F);
 
+/* One-electron integrals of n primitive pairs (unnormalized primitives
+   x_A^i y_A^j z_A^k exp(-a r_A^2), Cartesian exponents up to a total of 10 per
+   primitive), one value per pair: the reference's per-primitive engines
+   computePGTOrbOverlap (GaussianOrbitals.jl:147-179), the kinetic energy
+   -1/2 <a|nabla^2|b> (computePGTOrbCoordDiff, :295-363, with Samplers.jl's
+   -1/2), the one-body repulsion int a b / |r - C| without a charge
+   (computePGTOrbOneBodyRepulsion!, :478-522) and the multipole
+   <a|(x-C_x)^d_x (y-C_y)^d_y (z-C_z)^d_z|b> (:184-247; degree per axis in
+   [0, 4]).  A, B: [n][3] centres; a, b: [n] exponents; la, lb: [n][3]
+   Cartesian exponents; C: the point (kinds COULOMB and MULTIPOLE, else may be
+   NULL); degree: [3] (MULTIPOLE, else may be NULL). */
+#define QBX_PRIM_OVERLAP 0
+#define QBX_PRIM_KINETIC 1
+#define QBX_PRIM_COULOMB 2
+#define QBX_PRIM_MULTIPOLE 3
+int  qbx_prim_one_e(qbx_ctx* ctx, int32_t kind, int32_t n, const double* A, const double* a, const int32_t* la,
+                    const double* B, const double* b, const int32_t* lb, const double* C, const int32_t* degree,
+                    double* out);
+
 /* Mixed-precision cut: quartets whose Schwarz bound exceeds `cut` run in FP64
    under QBX_PREC_MIXED (default 1e-5). */
 int  qbx_set_mixed_cut(qbx_ctx* ctx, double cut);

@@ -293,7 +293,7 @@ void build_boys_table(std::vector<double>& tab, std::vector<double>& et, std::ve
     rows.assign((size_t)(QBX_BOYSL_LMAX + 1) * QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE, 0.0);
     long double F[32];
     static_assert(QBX_BOYS_NCOL <= 32, "one-electron Boys table orders too high");
-    // one-electron kernels and the generic ERI kernel: F_0..F_24 on the 0.1 grid
+    // one-electron kernels, the primitive kernel and the generic ERI kernel: F_0..F_28 on the 0.1 grid
     for (int i = 0; i < QBX_BOYS_ROWS; ++i) {
         const long double e = boys_seq_ld((long double)i * QBX_BOYS_DT, QBX_BOYS_NCOL - 1, F);
         for (int m = 0; m < QBX_BOYS_NCOL; ++m) tab[(size_t)i * QBX_BOYS_NCOL + m] = (double)F[m];
@@ -373,6 +373,111 @@ __device__ double coulomb1_comp_t(const int* la, const int* lb, const double* PA
     return vals[0];
 }
 
+
+// ---------------------------------------------------------------------------
+// kernel: one-electron integrals of primitive pairs (the reference's
+// per-primitive engines), any angular momentum up to kPrimLMax per primitive
+// ---------------------------------------------------------------------------
+constexpr int kPrimLMax = 10;
+constexpr int kPrimOv = kPrimLMax + 5;   // per-axis tables: j up to l_b + 4 (multipole degree) or l_b + 2 (kinetic)
+
+struct PrimOneArgs {
+    int n;
+    int kind;                  // ONE_OVERLAP, ONE_KINETIC, ONE_NUCLEAR (1/|r - C|, no charge), ONE_MULTIPOLE
+    const double* A;           // [n][3] centres and exponents of the first primitives
+    const double* a;
+    const int* la;             // [n][3] Cartesian exponents
+    const double* B;
+    const double* b;
+    const int* lb;
+    double C[3];
+    int deg[3];
+    const double* boys;
+    const double* bexp;
+    double* out;
+};
+
+// F_0..F_n for any T >= 0 and n <= QBX_BOYS_NMAX: the table below its range;
+// above it F_0 = sqrt(pi/4T) and the upward recurrence with its e^{-T} term,
+// F_{m+1} = ((2m+1) F_m - e^{-T}) / 2T (stable for T > m).
+__device__ void prim_boys(int n, double T, double* F, const double* boys, const double* bexp)
+{
+    if (T < QBX_BOYS_TMAX) {
+        qbx_boys_rt(n, T, F, boys, bexp);
+        return;
+    }
+    const double e = exp(-T), r = 0.5 / T;
+    F[0] = 0.88622692545275801365 * rsqrt(T);
+    for (int m = 0; m < n; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * r;
+}
+
+__global__ __launch_bounds__(256) void prim_one_e_kernel(PrimOneArgs g)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= g.n) return;
+    const double a = g.a[i], b = g.b[i], p = a + b, inv2p = 0.5 / p;
+    int la[3], lb[3];
+    double A[3], B[3], AB[3], PA[3], PB[3];
+    double r2 = 0.0;
+    for (int t = 0; t < 3; ++t) {
+        la[t] = g.la[3 * i + t];
+        lb[t] = g.lb[3 * i + t];
+        A[t] = g.A[3 * i + t];
+        B[t] = g.B[3 * i + t];
+        AB[t] = A[t] - B[t];
+        r2 += AB[t] * AB[t];
+        const double P = (a * A[t] + b * B[t]) / p;
+        PA[t] = P - A[t];
+        PB[t] = P - B[t];
+    }
+    const double kab = exp(-a * b / p * r2);
+    double val;
+    if (g.kind == ONE_NUCLEAR) {
+        const int Lt = la[0] + la[1] + la[2] + lb[0] + lb[1] + lb[2];
+        double PC[3];
+        for (int t = 0; t < 3; ++t) PC[t] = PA[t] + A[t] - g.C[t];
+        const double T = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+        double F[2 * kPrimLMax + 1];
+        prim_boys(Lt, T, F, g.boys, g.bexp);
+        val = 2.0 * kPi / p * kab * coulomb1_comp_t<2 * kPrimLMax>(la, lb, PA, PC, AB, inv2p, F, Lt);
+    } else {
+        const int extra = g.kind == ONE_KINETIC ? 2 : 0;
+        double f[3], s0[3];
+        for (int t = 0; t < 3; ++t) {
+            const int nj = lb[t] + (g.kind == ONE_MULTIPOLE ? g.deg[t] : extra);
+            double S[kPrimOv][kPrimOv];
+            qbx_ov1d_t<kPrimOv>(PA[t], PB[t], inv2p, la[t], nj, S);
+            const int ii = la[t], j = lb[t];
+            if (g.kind == ONE_MULTIPOLE) {
+                // (x - M)^d = sum_k C(d, k) (x - B)^k (B - M)^(d - k)
+                const int d = g.deg[t];
+                const double bm = B[t] - g.C[t];
+                double m = 0.0, c = 1.0;
+                for (int k = 0; k <= d; ++k) {
+                    double pw = 1.0;
+                    for (int q = 0; q < d - k; ++q) pw *= bm;
+                    m += c * pw * S[ii][j + k];
+                    c = c * (d - k) / (k + 1);
+                }
+                f[t] = m;
+            } else if (g.kind == ONE_KINETIC) {
+                // d^2/dx^2 acting on the second primitive
+                double r = -2.0 * b * (2 * j + 1) * S[ii][j] + 4.0 * b * b * S[ii][j + 2];
+                if (j > 1) r += (double)(j * (j - 1)) * S[ii][j - 2];
+                f[t] = r;
+                s0[t] = S[ii][j];
+            } else {
+                f[t] = S[ii][j];
+            }
+        }
+        const double pre = 5.568327996831707845 * kab / (p * sqrt(p));   // (pi / p)^(3/2) e^{-ab/p r^2}
+        if (g.kind == ONE_KINETIC)
+            val = pre * -0.5 * (f[0] * s0[1] * s0[2] + s0[0] * f[1] * s0[2] + s0[0] * s0[1] * f[2]);
+        else
+            val = pre * f[0] * f[1] * f[2];
+    }
+    g.out[i] = val;
+}
 
 // ---------------------------------------------------------------------------
 // kernel: one-electron part of the RHF energy gradient
@@ -2423,6 +2528,71 @@ int qbx_boys_device(qbx_ctx* ctx, int32_t order, int32_t n, const double* T, dou
         dF.ensure((size_t)n * (order + 1));
         boys_probe(ctx->boysl.p + (size_t)order * QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE, order, dT.p, n, dF.p, st);
         QBX_HIP(hipMemcpyAsync(F, dF.p, (size_t)n * (order + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+        QBX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int qbx_prim_one_e(qbx_ctx* ctx, int32_t kind, int32_t n, const double* A, const double* a, const int32_t* la,
+                   const double* B, const double* b, const int32_t* lb, const double* C, const int32_t* degree,
+                   double* out)
+{
+    return guarded([&] {
+        if (!ctx) throw QbxError(QBX_EINVAL, "null context");
+        if (kind < QBX_PRIM_OVERLAP || kind > QBX_PRIM_MULTIPOLE) throw QbxError(QBX_EINVAL, "unknown integral kind");
+        if (n < 0) throw QbxError(QBX_EINVAL, "n must be non-negative");
+        if (n == 0) return;
+        if (!A || !a || !la || !B || !b || !lb || !out) throw QbxError(QBX_EINVAL, "null argument or output");
+        if ((kind == QBX_PRIM_COULOMB || kind == QBX_PRIM_MULTIPOLE) && !C) throw QbxError(QBX_EINVAL, "null point");
+        PrimOneArgs g{};
+        g.n = n;
+        g.kind = kind == QBX_PRIM_OVERLAP ? ONE_OVERLAP
+                 : kind == QBX_PRIM_KINETIC ? ONE_KINETIC
+                 : kind == QBX_PRIM_COULOMB ? ONE_NUCLEAR
+                                            : ONE_MULTIPOLE;
+        if (kind == QBX_PRIM_MULTIPOLE) {
+            if (!degree) throw QbxError(QBX_EINVAL, "null multipole degree");
+            for (int t = 0; t < 3; ++t) {
+                if (degree[t] < 0 || degree[t] > 4) throw QbxError(QBX_EUNSUP, "multipole degree per axis must be in [0, 4]");
+                g.deg[t] = degree[t];
+            }
+        }
+        if (C)
+            for (int t = 0; t < 3; ++t) g.C[t] = C[t];
+        for (int i = 0; i < n; ++i) {
+            if (!(a[i] > 0.0) || !(b[i] > 0.0)) throw QbxError(QBX_EINVAL, "exponents must be positive");
+            int sa = 0, sb = 0;
+            for (int t = 0; t < 3; ++t) {
+                if (la[3 * i + t] < 0 || lb[3 * i + t] < 0) throw QbxError(QBX_EINVAL, "negative angular momentum");
+                sa += la[3 * i + t];
+                sb += lb[3 * i + t];
+            }
+            if (sa > kPrimLMax || sb > kPrimLMax)
+                throw QbxError(QBX_EUNSUP, "primitive angular momentum above " + std::to_string(kPrimLMax));
+        }
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        QBX_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        DevBuf<double> dA, da, dB, db, dout;
+        DevBuf<int> dla, dlb;
+        dA.upload(std::vector<double>(A, A + 3 * (size_t)n), st);
+        da.upload(std::vector<double>(a, a + n), st);
+        dB.upload(std::vector<double>(B, B + 3 * (size_t)n), st);
+        db.upload(std::vector<double>(b, b + n), st);
+        dla.upload(std::vector<int>(la, la + 3 * (size_t)n), st);
+        dlb.upload(std::vector<int>(lb, lb + 3 * (size_t)n), st);
+        dout.ensure(n);
+        g.A = dA.p;
+        g.a = da.p;
+        g.B = dB.p;
+        g.b = db.p;
+        g.la = dla.p;
+        g.lb = dlb.p;
+        g.boys = ctx->boys.p;
+        g.bexp = ctx->bexp.p;
+        g.out = dout.p;
+        hipLaunchKernelGGL(prim_one_e_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g);
+        QBX_HIP(hipGetLastError());
+        QBX_HIP(hipMemcpyAsync(out, dout.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
         QBX_HIP(hipStreamSynchronize(st));
     });
 }

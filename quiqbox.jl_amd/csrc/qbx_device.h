@@ -17,7 +17,7 @@
 #define QBX_BOYS_INVDT 10.0
 #define QBX_BOYS_TMAX 60.0
 #define QBX_BOYS_ROWS 602
-#define QBX_BOYS_NCOL 25     // F_0 .. F_24: highest order 17 (g quartets need 16) plus 7 Taylor orders above it
+#define QBX_BOYS_NCOL 29     // F_0 .. F_28: highest order 21 (g quartets need 16, two l = 10 primitives 20) plus 7 Taylor orders above it
 #define QBX_BOYS_TAYLOR 8
 #define QBX_BOYS_NMAX (QBX_BOYS_NCOL - QBX_BOYS_TAYLOR)
 // Per-order rows for the ERI kernels (staged in LDS, 23 KB): on the grid
@@ -127,10 +127,10 @@ __host__ __device__ inline void qbx_cart(int L, int c, int* ijk)
 }
 
 // 1D overlap recurrence (Obara-Saika): table S[i][j] of
-// int (x-A)^i (x-B)^j exp(-p (x-P)^2) dx / sqrt(pi/p),  i <= ni, j <= nj.
+// int (x-A)^i (x-B)^j exp(-p (x-P)^2) dx / sqrt(pi/p),  i <= ni, j <= nj < M.
 #define QBX_OV_MAX 9
-__device__ inline void qbx_ov1d(double PA, double PB, double inv2p, int ni, int nj,
-                                double S[QBX_OV_MAX][QBX_OV_MAX])
+template <int M>
+__device__ inline void qbx_ov1d_t(double PA, double PB, double inv2p, int ni, int nj, double S[M][M])
 {
     S[0][0] = 1.0;
     for (int i = 0; i <= ni; ++i) {
@@ -146,4 +146,9 @@ __device__ inline void qbx_ov1d(double PA, double PB, double inv2p, int ni, int 
             S[i][j] = v;
         }
     }
+}
+
+__device__ inline void qbx_ov1d(double PA, double PB, double inv2p, int ni, int nj, double S[QBX_OV_MAX][QBX_OV_MAX])
+{
+    qbx_ov1d_t<QBX_OV_MAX>(PA, PB, inv2p, ni, nj, S);
 }

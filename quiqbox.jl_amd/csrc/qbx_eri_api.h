@@ -122,7 +122,7 @@ struct GenArgs {
     const int* ppk;           // radial indices (k_a | k_b << 16) of every primitive pair
     const int* poff;          // first radial primitive of every shell
     const double* pexp;       // radial exponents
-    const double* boys;       // one-electron Boys table (F_0..F_24 on the 0.1 grid)
+    const double* boys;       // one-electron Boys table (F_0..F_28 on the 0.1 grid)
     const double* bexp;
     int nbra;
     int64_t total;            // quartets (waves)

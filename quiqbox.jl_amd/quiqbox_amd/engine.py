@@ -53,6 +53,31 @@ class Context:
             _lib.check(_lib.load().qbx_boys_device(self.h, int(order), len(T), _lib.dp(T), _lib.dp(F)))
         return F
 
+    def prim_one_e(self, kind, A, a, la, B, b, lb, C=None, degree=None):
+        """One-electron integrals of primitive pairs on the device
+        (qbx_prim_one_e): kind QBX_PRIM_OVERLAP / KINETIC / COULOMB (one-body
+        repulsion at C, no charge) / MULTIPOLE (about C, degree per axis);
+        A, B (n, 3) centres, a, b (n,) exponents, la, lb (n, 3) Cartesian
+        exponents.  Returns an (n,) array."""
+        A = np.ascontiguousarray(np.atleast_2d(A), dtype=np.float64)
+        B = np.ascontiguousarray(np.atleast_2d(B), dtype=np.float64)
+        a = np.ascontiguousarray(np.atleast_1d(a), dtype=np.float64)
+        b = np.ascontiguousarray(np.atleast_1d(b), dtype=np.float64)
+        la = np.ascontiguousarray(np.atleast_2d(la), dtype=np.int32)
+        lb = np.ascontiguousarray(np.atleast_2d(lb), dtype=np.int32)
+        n = len(a)
+        if not (A.shape == B.shape == la.shape == lb.shape == (n, 3) and len(b) == n):
+            raise ValueError("DimensionMismatch: primitive-pair arrays of different lengths")
+        Cv = None if C is None else np.ascontiguousarray(C, dtype=np.float64)
+        Cp = None if Cv is None else _lib.dp(Cv)
+        dv = None if degree is None else np.ascontiguousarray(degree, dtype=np.int32)
+        out = np.zeros(n)
+        with self.lock:
+            _lib.check(_lib.load().qbx_prim_one_e(self.h, int(kind), n, _lib.dp(A), _lib.dp(a), _lib.ip(la),
+                                                  _lib.dp(B), _lib.dp(b), _lib.ip(lb), Cp,
+                                                  None if dv is None else _lib.ip(dv), _lib.dp(out)))
+        return out
+
     def set_mixed_cut(self, cut):
         _lib.check(_lib.load().qbx_set_mixed_cut(self.h, float(cut)))
 
