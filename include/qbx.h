@@ -182,6 +182,13 @@ int  qbx_prim_one_e(qbx_ctx* ctx, int32_t kind, int32_t n, const double* A, cons
                     const double* B, const double* b, const int32_t* lb, const double* C, const int32_t* degree,
                     double* out);
 
+/* Electron-repulsion integrals (ab|cd) of n primitive quartets with unit
+   coefficients, one value each: the reference's per-primitive engine
+   computePGTOrbTwoBodyRepulsion! (GaussianOrbitals.jl:594-663).  cen: [n][4][3]
+   centres, xpn: [n][4] exponents, ang: [n][4][3] Cartesian exponents, in the
+   order a, b, c, d; at most 10 per primitive and 24 per quartet. */
+int  qbx_prim_eri(qbx_ctx* ctx, int32_t n, const double* cen, const double* xpn, const int32_t* ang, double* out);
+
 /* Mixed-precision cut: quartets whose Schwarz bound exceeds `cut` run in FP64
    under QBX_PREC_MIXED (default 1e-5). */
 int  qbx_set_mixed_cut(qbx_ctx* ctx, double cut);

@@ -291,9 +291,9 @@ void build_boys_table(std::vector<double>& tab, std::vector<double>& et, std::ve
     tab.assign((size_t)QBX_BOYS_ROWS * QBX_BOYS_NCOL, 0.0);
     et.assign(QBX_BOYS_ROWS, 0.0);
     rows.assign((size_t)(QBX_BOYSL_LMAX + 1) * QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE, 0.0);
-    long double F[32];
-    static_assert(QBX_BOYS_NCOL <= 32, "one-electron Boys table orders too high");
-    // one-electron kernels, the primitive kernel and the generic ERI kernel: F_0..F_28 on the 0.1 grid
+    long double F[64];
+    static_assert(QBX_BOYS_NCOL <= 64, "one-electron Boys table orders too high");
+    // one-electron kernels, the primitive kernels and the generic ERI kernel: F_0..F_32 on the 0.1 grid
     for (int i = 0; i < QBX_BOYS_ROWS; ++i) {
         const long double e = boys_seq_ld((long double)i * QBX_BOYS_DT, QBX_BOYS_NCOL - 1, F);
         for (int m = 0; m < QBX_BOYS_NCOL; ++m) tab[(size_t)i * QBX_BOYS_NCOL + m] = (double)F[m];
@@ -2592,6 +2592,45 @@ int qbx_prim_one_e(qbx_ctx* ctx, int32_t kind, int32_t n, const double* A, const
         g.out = dout.p;
         hipLaunchKernelGGL(prim_one_e_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g);
         QBX_HIP(hipGetLastError());
+        QBX_HIP(hipMemcpyAsync(out, dout.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+        QBX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int qbx_prim_eri(qbx_ctx* ctx, int32_t n, const double* cen, const double* xpn, const int32_t* ang, double* out)
+{
+    return guarded([&] {
+        if (!ctx) throw QbxError(QBX_EINVAL, "null context");
+        if (n < 0) throw QbxError(QBX_EINVAL, "n must be non-negative");
+        if (n == 0) return;
+        if (!cen || !xpn || !ang || !out) throw QbxError(QBX_EINVAL, "null argument or output");
+        for (int i = 0; i < n; ++i) {
+            int lt = 0;
+            for (int k = 0; k < 4; ++k) {
+                if (!(xpn[4 * i + k] > 0.0)) throw QbxError(QBX_EINVAL, "exponents must be positive");
+                int lk = 0;
+                for (int t = 0; t < 3; ++t) {
+                    if (ang[12 * i + 3 * k + t] < 0) throw QbxError(QBX_EINVAL, "negative angular momentum");
+                    lk += ang[12 * i + 3 * k + t];
+                }
+                if (lk > kPrimLMax)
+                    throw QbxError(QBX_EUNSUP, "primitive angular momentum above " + std::to_string(kPrimLMax));
+                lt += lk;
+            }
+            if (lt > QBX_PRIM_ERI_LTOT)
+                throw QbxError(QBX_EUNSUP, "total angular momentum of a primitive quartet above " +
+                                               std::to_string(QBX_PRIM_ERI_LTOT));
+        }
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        QBX_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        DevBuf<double> dc, dx, dout;
+        DevBuf<int> da;
+        dc.upload(std::vector<double>(cen, cen + 12 * (size_t)n), st);
+        dx.upload(std::vector<double>(xpn, xpn + 4 * (size_t)n), st);
+        da.upload(std::vector<int>(ang, ang + 12 * (size_t)n), st);
+        dout.ensure(n);
+        prim_eri(n, dc.p, dx.p, da.p, ctx->boys.p, ctx->bexp.p, dout.p, st);
         QBX_HIP(hipMemcpyAsync(out, dout.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
         QBX_HIP(hipStreamSynchronize(st));
     });

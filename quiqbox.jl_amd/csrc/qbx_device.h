@@ -17,7 +17,7 @@
 #define QBX_BOYS_INVDT 10.0
 #define QBX_BOYS_TMAX 60.0
 #define QBX_BOYS_ROWS 602
-#define QBX_BOYS_NCOL 29     // F_0 .. F_28: highest order 21 (g quartets need 16, two l = 10 primitives 20) plus 7 Taylor orders above it
+#define QBX_BOYS_NCOL 33     // F_0 .. F_32: highest order 25 (g quartets need 16, primitive quartets up to 24) plus 7 Taylor orders above it
 #define QBX_BOYS_TAYLOR 8
 #define QBX_BOYS_NMAX (QBX_BOYS_NCOL - QBX_BOYS_TAYLOR)
 // Per-order rows for the ERI kernels (staged in LDS, 23 KB): on the grid

@@ -122,7 +122,7 @@ struct GenArgs {
     const int* ppk;           // radial indices (k_a | k_b << 16) of every primitive pair
     const int* poff;          // first radial primitive of every shell
     const double* pexp;       // radial exponents
-    const double* boys;       // one-electron Boys table (F_0..F_28 on the 0.1 grid)
+    const double* boys;       // one-electron Boys table (F_0..F_32 on the 0.1 grid)
     const double* bexp;
     int nbra;
     int64_t total;            // quartets (waves)
@@ -153,6 +153,13 @@ struct GenArgs {
 // nm: the per-axis recurrence size (4 x the highest shell angular momentum
 // of the quartets, 12 or 16); mode: MODE_JK, MODE_DENSE or MODE_SCHWARZ
 void launch_eri_generic(const GenArgs& a, int nm, int mode, hipStream_t s);
+
+// Primitive-quartet ERIs with unit coefficients (qbx_eri_generic.hip), one
+// thread each: cen [n][4][3], xpn [n][4], ang [n][4][3] (order a, b, c, d of
+// (ab|cd)), total angular momentum at most QBX_PRIM_ERI_LTOT.
+#define QBX_PRIM_ERI_LTOT 24
+void prim_eri(int n, const double* cen, const double* xpn, const int* ang, const double* boys, const double* bexp,
+              double* out, hipStream_t s);
 
 // The ERI kernels' Boys function alone (qbx_boys_probe.hip): F_0..F_L at n
 // device arguments T from the order-L table rows, into F (n x (L+1)).

@@ -314,7 +314,56 @@ void launch_nm(const GenArgs& a, int mode, hipStream_t s)
     QBX_HIP(hipGetLastError());
 }
 
+// Primitive quartets one thread each (unit coefficients): the reference's
+// per-primitive engine computePGTOrbTwoBodyRepulsion! (GaussianOrbitals.jl:594-663).
+__global__ __launch_bounds__(64) void prim_eri_kernel(int n, const double* __restrict__ cen,
+                                                      const double* __restrict__ xpn, const int* __restrict__ ang,
+                                                      const double* boys, const double* bexp, double* __restrict__ out)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    int l[4][3];
+    double c[4][3], e[4];
+    for (int k = 0; k < 4; ++k) {
+        e[k] = xpn[4 * i + k];
+        for (int t = 0; t < 3; ++t) {
+            c[k][t] = cen[12 * i + 3 * k + t];
+            l[k][t] = ang[12 * i + 3 * k + t];
+        }
+    }
+    PrimGeom G[2];
+    for (int h = 0; h < 2; ++h) {
+        const int ka = 2 * h, kb = 2 * h + 1;
+        PrimGeom& g = G[h];
+        g.ea = e[ka];
+        g.eb = e[kb];
+        g.p = e[ka] + e[kb];
+        double r2 = 0.0, P[3];
+        for (int t = 0; t < 3; ++t) {
+            g.AB[t] = c[ka][t] - c[kb][t];
+            r2 += g.AB[t] * g.AB[t];
+            P[t] = (e[ka] * c[ka][t] + e[kb] * c[kb][t]) / g.p;
+            g.PA[t] = P[t] - c[ka][t];
+        }
+        g.Px = P[0];
+        g.Py = P[1];
+        g.Pz = P[2];
+        g.K = exp(-e[ka] * e[kb] / g.p * r2) / g.p;
+    }
+    out[i] = prim_quartet<QBX_PRIM_ERI_LTOT>(G[0], G[1], l[0], l[1], l[2], l[3], boys, bexp);
+}
+
 }  // namespace
+
+void prim_eri(int n, const double* cen, const double* xpn, const int* ang, const double* boys, const double* bexp,
+              double* out, hipStream_t s)
+{
+    static_assert(QBX_PRIM_ERI_LTOT <= QBX_BOYS_NMAX, "Boys table orders for the primitive ERIs");
+    if (n <= 0) return;
+    hipLaunchKernelGGL(prim_eri_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, cen, xpn, ang, boys, bexp,
+                       out);
+    QBX_HIP(hipGetLastError());
+}
 
 void launch_eri_generic(const GenArgs& a, int nm, int mode, hipStream_t s)
 {

@@ -12,7 +12,8 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libqbx_hip.
 
 QBX_SHELL_MAXC = 15     # Cartesian components of the largest shell (g), qbx.h
 QBX_MAX_L = 4
-QBX_PRIM_LMAX = 10          # qbx_prim_one_e: total Cartesian exponent per primitive
+QBX_PRIM_LMAX = 10          # qbx_prim_one_e, qbx_prim_eri: total Cartesian exponent per primitive
+QBX_PRIM_ERI_LTOT = 24      # qbx_prim_eri: total over the quartet
 QBX_PRIM_OVERLAP, QBX_PRIM_KINETIC, QBX_PRIM_COULOMB, QBX_PRIM_MULTIPOLE = 0, 1, 2, 3
 QBX_PREC_FP64 = 0
 QBX_PREC_MIXED = 1
@@ -51,6 +52,7 @@ SIGNATURES = {
     "qbx_set_density_screening": (_i32, [_vp, _d]),
     "qbx_boys_device": (_i32, [_vp, _i32, _i32, _pd, _pd]),
     "qbx_prim_one_e": (_i32, [_vp, _i32, _i32, _pd, _pd, _pi, _pd, _pd, _pi, _pd, _pi, _pd]),
+    "qbx_prim_eri": (_i32, [_vp, _i32, _pd, _pd, _pi, _pd]),
     "qbx_basis_shells": (_i32, [_vp, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd]),
     "qbx_rhf_gradient": (_i32, [_vp, _vp, _pd, _pd, _i32, _pd, _pd, _d, _pd, _pd, _pd]),
     "qbx_density_purify": (_i32, [_vp, _i32, _i32, _pd, _pd, _pd, _i32, _pi, _pd]),

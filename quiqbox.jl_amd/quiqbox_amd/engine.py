@@ -78,6 +78,19 @@ class Context:
                                                   None if dv is None else _lib.ip(dv), _lib.dp(out)))
         return out
 
+    def prim_eri(self, quartets):
+        """(ab|cd) of primitive quartets with unit coefficients on the device
+        (qbx_prim_eri): quartets is a list of four (centre, exponent,
+        (i, j, k)) primitives each.  Returns an (n,) array."""
+        n = len(quartets)
+        cen = np.ascontiguousarray([[p[0] for p in q] for q in quartets], dtype=np.float64).reshape(n, 4, 3)
+        xpn = np.ascontiguousarray([[p[1] for p in q] for q in quartets], dtype=np.float64).reshape(n, 4)
+        ang = np.ascontiguousarray([[p[2] for p in q] for q in quartets], dtype=np.int32).reshape(n, 4, 3)
+        out = np.zeros(n)
+        with self.lock:
+            _lib.check(_lib.load().qbx_prim_eri(self.h, n, _lib.dp(cen), _lib.dp(xpn), _lib.ip(ang), _lib.dp(out)))
+        return out
+
     def set_mixed_cut(self, cut):
         _lib.check(_lib.load().qbx_set_mixed_cut(self.h, float(cut)))
 

@@ -1,17 +1,22 @@
-"""Primitive-pair one-electron integrals on the GPU (qbx_prim_one_e): the
+"""Primitive-level integrals on the GPU (qbx_prim_one_e, qbx_prim_eri): the
 reference's per-primitive engines at angular momentum up to 10 per primitive.
 
 - The reference's own literals (tests/golden/reference_literals.json): the
   one-body repulsion values of Coulomb-test.jl:27-40, including the l = 10
   primitive (1, 7, 2), and the primitive kinetic value of Kinetic-test.jl.
+- The reference's primitive ERI literals (Coulomb-test.jl:51-81, total
+  angular momentum up to 24, the l = 10 primitive in four positions).
 - Random primitive pairs with every total angular momentum 0..10 against the
   oracle's primitive routines (overlap, kinetic, one-body repulsion,
-  multipoles), relative to the magnitude of the value."""
+  multipoles), and random primitive quartets up to l = 10 each and 24 in
+  total against the oracle's primitive ERI, relative to the magnitude of the
+  value."""
 import numpy as np
 import pytest
 
 import oracle
 import quiqbox_amd as qb
+from exact_eri import primitive_eri
 from helpers import literals
 from quiqbox_amd import _lib
 from quiqbox_amd.engine import default_context
@@ -103,3 +108,46 @@ def test_rejects_angular_momentum_above_limit():
     p = ((0.0, 0.0, 0.0), 1.0, (11, 0, 0))
     with pytest.raises(Exception):
         _run(_lib.QBX_PRIM_OVERLAP, [(p, p)])
+
+
+def test_reference_primitive_eri_literals():
+    lit = literals()["eri_prim"]
+    quartets = [_prims(q) for q, _ in lit["values"]]
+    ref = np.array([v for _, v in lit["values"]])
+    got = default_context().prim_eri(quartets)
+    # exact values (40-digit McMurchie-Davidson): the reference's own literals
+    # are up to 4.7e-13 off them at l = 10
+    exact = np.array([float(primitive_eri(*q)) for q in quartets])
+    for (q, v), g, x in zip(lit["values"], got, exact):
+        _say("(%d %d|%d %d): gpu %.16e reference %.16e exact %.16e" % (tuple(q) + (g, v, x)))
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(got, exact, rtol=2e-13, atol=0)
+
+
+def test_random_primitive_quartets_against_exact_values():
+    rng = np.random.default_rng(9)
+    quartets = []
+    for k in range(96):
+        Ls = [int(x) for x in rng.integers(0, 11, 4)]
+        while sum(Ls) > 24:
+            Ls[int(np.argmax(Ls))] -= 1
+        q = []
+        for L in Ls:
+            i = int(rng.integers(0, L + 1))
+            j = int(rng.integers(0, L - i + 1))
+            q.append((tuple(rng.uniform(-1.0, 1.0, 3)), float(np.exp(rng.uniform(np.log(0.3), np.log(4.0)))),
+                      (i, j, L - i - j)))
+        quartets.append(q)
+    got = default_context().prim_eri(quartets)
+    # against exact values: at these angular momenta the oracle's own
+    # recurrence loses up to 1.3e-8 relative (quartet 4, l = 6, 7, 5, 6)
+    exact = np.array([float(primitive_eri(*q)) for q in quartets])
+    ref = np.array([oracle.prim_eri(*q) for q in quartets])
+    scale = np.max(np.abs(exact))
+    rel = np.abs(got - exact) / np.maximum(np.abs(exact), 1e-6 * scale)
+    rel_o = np.abs(ref - exact) / np.maximum(np.abs(exact), 1e-6 * scale)
+    k = int(np.argmax(rel))
+    _say("primitive ERIs: %d quartets (l up to 10 each, 24 total), worst rel to exact %.2e at l %s (gpu %.6e exact "
+         "%.6e); the oracle's worst %.2e" % (len(quartets), rel[k], [sum(p[2]) for p in quartets[k]], got[k],
+                                             exact[k], float(np.max(rel_o))))
+    assert rel[k] < 1e-10
