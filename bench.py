@@ -269,6 +269,38 @@ def learned_section(nuc, gb, res64, t64, n_train, epochs):
     return out
 
 
+# Unseen clusters of the random C/H/O generator (s+p shells, cho_cluster) for
+# the packaged network: the six held-out clusters of its GPU test and the
+# three of the 24 evaluation clusters (seeds 916000-916007, 924000-924007,
+# 932000-932007) on which any starting guess converged in 150 steps.
+CHO_EVAL = [(16, 16010), (16, 16012), (16, 16013), (16, 16015), (24, 24010), (24, 24012),
+            (16, 916002), (16, 916006), (24, 924004)]
+
+
+def learned_cho_section():
+    """The packaged C/H/O network (trained on GPU-converged densities of
+    8-16-atom clusters of the random C/H/O generator) against the
+    core-Hamiltonian guess and Quiqbox's SAD guess on unseen clusters of that
+    generator: convergence in 150 steps, Fock builds and wall time."""
+    from quiqbox_amd import learned_guess as lg
+    from quiqbox_amd import train_guess as tg
+    t0 = time.perf_counter()
+    model = lg.load_cho_model()
+    guesses = ["CoreH", "learned", "SAD"]
+    rows = tg.evaluate_clusters(model, CHO_EVAL, sad=True, say=log)
+    out = tg.summarize(rows, guesses)
+    out["model"] = {k: model.meta[k] for k in ("systems", "converged", "pairs", "atoms", "epochs", "loss_last")}
+    out["per_cluster"] = [{"atoms": r["atoms"], "seed": r["seed"],
+                           **{g: [r[g]["converged"], r[g]["builds"], round(r[g]["wall_s"], 3)] for g in guesses}}
+                          for r in rows]
+    out["section_s"] = time.perf_counter() - t0
+    log("C/H/O learned guess: %s; over the %d clusters on which every guess converged: %s" % (
+        ", ".join("%s converged %d/%d" % (g, out[g]["converged"], out["clusters"]) for g in guesses),
+        out["all_converged"], ", ".join("%s %d builds %.2f s" % (g, out[g]["builds_all_converged"],
+                                                                out[g]["wall_s_all_converged"]) for g in guesses)))
+    return out
+
+
 def gradient_section(steps):
     """Basis-parameter gradient on the 32-atom H cluster with floating
     4-primitive s and p shells (256 exponents, 256 coefficients, 96 centre
@@ -404,6 +436,8 @@ def main():
     ap.add_argument("--learned-systems", type=int, default=40,
                     help="training systems of the learned-guess section (0: skip it)")
     ap.add_argument("--learned-epochs", type=int, default=300)
+    ap.add_argument("--no-cho", action="store_true",
+                    help="skip the packaged C/H/O network's evaluation on unseen C/H/O clusters")
     ap.add_argument("--grad-steps", type=int, default=100,
                     help="basis-optimization steps of the gradient section (0: skip it)")
     ap.add_argument("--mo-size", type=int, default=128,
@@ -534,6 +568,8 @@ def main():
         out["rhf"] = rhf_section(hip, min(args.cpu_seconds, 8.0), args.no_cpu)
         if args.learned_systems > 0:
             out["learned_guess"] = learned_section(*rhf_section.last, args.learned_systems, args.learned_epochs)
+        if not args.no_cho:
+            out["learned_guess_cho"] = learned_cho_section()
         if args.grad_steps > 0:
             out["gradient"] = gradient_section(args.grad_steps)
     if args.mo_size > 0:

@@ -28,6 +28,7 @@ SAD, the other plain guess, runs one full-basis UHF per nucleus,
 `HartreeFock.jl:256-293`: minutes at 600 functions, so it is not the fallback.)
 """
 import ctypes
+import os
 import time
 
 import numpy as np
@@ -376,6 +377,19 @@ class LearnedGuess:
         G = gb.jk(Dt, 0.5 * Dt, schwarz)
         X = gb.orthonormalization()
         return getC(X, Hcore + G), Dt
+
+
+# The C/H/O network shipped with the package: trained by
+# `python -m quiqbox_amd.train_guess --systems 1600 --min-atoms 8 --max-atoms 16
+# --epochs 150 --batch 2048` on the GPU-converged RHF densities of random C/H/O
+# clusters (cho_training_systems, seeds 100000-101599: 801 of the 1600
+# converged, 110,726 atom pairs); the file's `meta` holds the counts and times.
+CHO_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "cho_sp_guess.npz")
+
+
+def load_cho_model():
+    """The packaged C/H/O network (CHO_SP layout) on the GPU."""
+    return LearnedGuess.load(CHO_MODEL)
 
 
 def guess_config(model, nuc, gb, strategy=None, maxStep=200, schwarz=1e-12, info=None):

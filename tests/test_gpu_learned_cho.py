@@ -62,3 +62,32 @@ def test_cho_learned_guess_on_unseen_clusters():
                                   "converged" if r1.converged else "not converged",
                                   r1.timing["fock_builds"], r1.energy[0] - r0.energy[0]))
     _say("converged (core guess, learned guess, learned used): %s" % outcome)
+
+
+def test_packaged_cho_network_on_unseen_clusters():
+    """The C/H/O network shipped with the package (quiqbox_amd/data/cho_sp_guess.npz,
+    trained by quiqbox_amd.train_guess on 1600 generated 8-16-atom clusters) loads
+    with its training record and gives a finite guess within 30% of the electron
+    count on every unseen cluster; the SCF outcomes next to the core guess are
+    printed (they are chaotic, as above)."""
+    model = lg.load_cho_model()
+    _say("packaged network: %s" % model.meta)
+    assert model.layout.elements == lg.CHO_SP.elements and model.layout.nmax == lg.CHO_SP.nmax
+    assert model.meta["systems"] == 1600 and model.meta["converged"] >= 400
+    assert model.meta["loss_last"] < 0.5 * model.meta["loss_first"]
+    strat = qb.SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-8))
+    for n, seed in HELD_OUT:
+        nuc, bs = lg.cho_cluster(n, seed)
+        gb = qb.GPUBasis(bs)
+        S, H, _ = lg.system_data(nuc, gb, model.layout)
+        C, Dt = model.initial_coefficients(nuc, gb, S, H)
+        assert np.all(np.isfinite(C)) and np.all(np.isfinite(Dt))
+        n_e = float(np.sum(nuc.charges))
+        n_pred = float(np.sum(Dt * S))
+        assert abs(n_pred - n_e) < 0.3 * n_e
+        info = {}
+        r1 = qb.runHartreeFock(nuc, gb, lg.guess_config(model, nuc, gb, strat, 150, info=info), printInfo=False)
+        assert np.isfinite(r1.energy[0])
+        _say("%d atoms seed %d: packaged guess (%s, %.1f of %.0f electrons) %s in %d builds" % (
+            n, seed, "used" if info["learned"] else "fallback", n_pred, n_e,
+            "converged" if r1.converged else "not converged", r1.timing["fock_builds"]))

@@ -67,3 +67,17 @@ def test_targets_and_assembly_are_inverse_block_maps():
             blk = Y[a * na + b].reshape(lg.NMAX, lg.NMAX)
             np.testing.assert_array_equal(blk[:len(blocks[a]), :len(blocks[b])], D[np.ix_(blocks[a], blocks[b])])
     np.testing.assert_allclose(lg.assemble_density(blocks, Y, n), D, rtol=0, atol=1e-15)
+
+
+def test_packaged_cho_network_file():
+    """The packaged C/H/O network is plain arrays (loads with allow_pickle=False)
+    whose shapes match the CHO_SP layout, with its training record."""
+    import json
+    z = np.load(lg.CHO_MODEL, allow_pickle=False)
+    L = lg.CHO_SP
+    assert tuple(str(e) for e in z["elements"]) == L.elements and int(z["nmax"]) == L.nmax
+    assert z["W0"].shape == (L.n_in, 256) and z["W1"].shape == (256, 256) and z["W2"].shape == (256, L.n_out)
+    assert z["mu"].shape == (L.n_feat,) and z["sd"].shape == (L.n_feat,)
+    assert all(np.all(np.isfinite(z[k])) for k in ("W0", "W1", "W2", "b0", "b1", "b2", "mu", "sd"))
+    meta = json.loads(str(z["meta"]))
+    assert meta["systems"] == 1600 and meta["atoms"] == [8, 16] and meta["pairs"] > 0
