@@ -74,11 +74,14 @@ COST_DIGEST = 8    # launch-order cost model: J/K digest per Cartesian quartet
 def ncart_prod(c):
     return functools.reduce(lambda x, l: x * (l + 1) * (l + 2) // 2, c, 1)
 # Classes whose staged kernel runs at one wave per SIMD with larger groups: the
-# register file of a lone wave (VGPRs plus AGPRs as spill space) holds 48
-# accumulators, so the class makes 21 passes over its primitive quartets
-# instead of 31.  Measured on the 592-function cluster: (dd|dd) 4.1 -> 2.7 ms;
-# the other staged classes lose more latency hiding than they save.
-STAGE_ONE_WAVE = {(2, 2, 2, 2): 48}
+# register file of a lone wave (VGPRs plus AGPRs as spill space) holds 62
+# accumulators (two bra components e), so the class makes 16 passes over its
+# primitive quartets instead of 31.  Measured on the 592-function cluster:
+# (dd|dd) 4.1 -> 2.7 ms at 21 passes (one wave per chunk); the other staged
+# classes lose more latency hiding than they save.  16 passes also split best
+# over the 4 waves that share a chunk (largest per-wave cost 6848 against 7386
+# at 21 passes, both with the assignment below).
+STAGE_ONE_WAVE = {(2, 2, 2, 2): 62}
 # Staged classes whose 64-ket chunks are each shared by the waves of a block
 # (value: waves per block; groups, then digest rows, spread over them; the
 # bra's K rows in LDS).  Measured serialized on the 592-function cluster,
@@ -88,6 +91,14 @@ STAGE_ONE_WAVE = {(2, 2, 2, 2): 48}
 # (dd|ds) 1.13 -> 1.27 ms lost (many tasks already fill the chip, and the
 # extra barriers cost).
 STAGE_SPLIT = {(2, 2, 2, 2): 4, (2, 2, 2, 1): 4, (2, 1, 2, 1): 4}
+# 64-ket chunks per task (one bra, its K rows flushed once per task) of the
+# classes of STAGE_SPLIT.  Each chunk already occupies a whole block, so small
+# tasks balance best.  Measured on the 592-function cluster against 8 chunks a
+# task, serialized: (dd|dd) 0.75 -> 0.60 ms at one chunk (0.72 at two), (dd|dp)
+# 2.55 -> 1.60 ms at two (1.45 at one), (dp|dp) 2.11 -> 1.87 ms at two (2.00 at
+# one); the concurrent build was 0.5-1 ms slower with one-chunk tasks for all
+# three than with two (more K-row flushes among the other classes' atomics).
+STAGE_SPLIT_CHUNKS = {(2, 2, 2, 2): 1, (2, 2, 2, 1): 2, (2, 1, 2, 1): 2}
 TWIN_MAX_ACC = 9   # must match qbx_eri_api.h
 
 
@@ -235,6 +246,7 @@ def gen_staged(la, lb, lc, ld, es, fs, eidx, fidx, C_, D_):
     sel = " : ".join("g == %d ? %d" % (k, bounds[k]) for k in range(G)) + " : %d" % nE
     code.append("    static constexpr int ge0(int g) { return %s; }" % sel)
     code.append("    static constexpr int GMAXACC = %d;" % (max(bounds[k + 1] - bounds[k] for k in range(G)) * nF))
+    gcost = []
     for g in range(G):
         eg = es[bounds[g]:bounds[g + 1]]
         em = Emitter()
@@ -244,6 +256,7 @@ def gen_staged(la, lb, lc, ld, es, fs, eidx, fidx, C_, D_):
         for (e, f) in targets:
             lines.append("acc[%d] += %s;" % ((eidx[e] - bounds[g]) * nF + fidx[f], vv[(e, f)]))
         flops_vrr_exec += em.flops + len(targets)
+        gcost.append(em.flops + len(targets) + COST_PRIM0)
         code.append("    static __device__ __forceinline__ void vrr_g%d(const R* __restrict__ F, R* __restrict__ acc," % g)
         code.append("        R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz,")
         code.append("        R WQx, R WQy, R WQz, R inv2p, R inv2q, R inv2pq, R rho_p, R rho_q) {")
@@ -280,6 +293,21 @@ def gen_staged(la, lb, lc, ld, es, fs, eidx, fidx, C_, D_):
                     "const W& ws, R CDx, R CDy, R CDz) {" % g)
         code += ["        " + l for l in hm.lines + outs]
         code.append("    }")
+    # Groups over the SPLIT waves of a block that share a chunk: longest
+    # processing time first on the per-primitive-quartet cost of each group's
+    # pass (its VRR plus the geometry and Boys values every pass repeats)
+    W = STAGE_SPLIT.get((la, lb, lc, ld), 1)
+    wave_of = [0] * G
+    load = [0] * W
+    for g in sorted(range(G), key=lambda g: -gcost[g]):
+        w = min(range(W), key=lambda w: load[w])
+        wave_of[g] = w
+        load[w] += gcost[g]
+    rr = [sum(gcost[g] for g in range(G) if g % W == w) for w in range(W)]
+    code.append("    // wave of a block (of SPLIT sharing a chunk) that evaluates group g; per-wave cost %s "
+                "(round robin: %s)" % (load, rr))
+    sel = " : ".join("g == %d ? %d" % (g, wave_of[g]) for g in range(G - 1)) + (" : %d" % wave_of[G - 1])
+    code.append("    static constexpr int grp_wave(int g) { return %s; }" % sel)
     args = "F, acc, PAx, PAy, PAz, WPx, WPy, WPz, QCx, QCy, QCz, WQx, WQy, WQz, inv2p, inv2q, inv2pq, rho_p, rho_q"
     code.append("    template <int G> static __device__ __forceinline__ void vrr_grp(const R* __restrict__ F, "
                 "R* __restrict__ acc,")
@@ -773,6 +801,13 @@ def main(outdir):
              "// workspace values per lane of the staged classes (0: not staged)",
              "static const int qbx_class_ws[%d] = {%s};" % (
                  len(classes), ", ".join(str(stage_ws(*c)) for c in classes)),
+             "// staged classes whose 64-ket chunks are shared by the waves of a block (waves per block; 1: not shared)",
+             "static const int qbx_class_split[%d] = {%s};" % (
+                 len(classes), ", ".join(str(STAGE_SPLIT.get(c, 1) if c in STAGED else 1) for c in classes)),
+             "// chunks per K-row task of the classes whose chunks are shared by a block's waves (0: not shared)",
+             "static const int qbx_class_split_chunks[%d] = {%s};" % (
+                 len(classes), ", ".join(str(STAGE_SPLIT_CHUNKS[c] if c in STAGED and STAGE_SPLIT.get(c, 1) > 1
+                                             else 0) for c in classes)),
              "// classes with a twin-bra kernel (not staged, at most %d accumulators)" % TWIN_MAX_ACC,
              "static const int qbx_class_twin[%d] = {%s};" % (
                  len(classes), ", ".join("1" if t else "0" for t in twin_ok)),

@@ -179,8 +179,9 @@ struct TaskSet {
     DevBuf<int64_t> prefix;
     DevBuf<int> koff;
     DevBuf<int> cpre;          // waves (chunks of 64 kets) per bra, prefix sums
-    DevBuf<int> spre;          // K-row blocks (ROW_CHUNKS chunks) per bra, prefix sums
+    DevBuf<int> spre;          // K-row blocks (chunks chunks) per bra, prefix sums
     int nsuper = 0;
+    int chunks = ROW_CHUNKS;   // chunks per K-row task
     int64_t total = 0;
     int nwave = 0;
     int nbra = 0;
@@ -1114,6 +1115,7 @@ struct TaskCounts {
     std::vector<int> w64, w32;   // waves of 64 kets per task range, prefix sums
     std::vector<int> s64, s32;   // K-row blocks per task range, prefix sums
     int ntb = 0;                 // task ranges (bras x buckets)
+    int rc = ROW_CHUNKS;         // chunks per K-row block
 };
 
 // Surviving kets of bra i in ket bucket k: FP64 [0, c64), FP32 [c64, cnt),
@@ -1138,6 +1140,14 @@ void bra_range(const qbx_basis* b, int cls, int i, int k, double thr, int mode, 
     if (mode == QBX_PREC_MIXED) c64 = std::min(cnt, qi > 0.0 ? ge(cut / qi) : 0);
 }
 
+// chunks of 64 kets per K-row task of class cls
+int task_chunks(int cls)
+{
+    if (qbx_class_ws[cls] == 0) return ROW_CHUNKS;
+    return qbx_class_split[cls] > 1 && qbx_class_split_chunks[cls] > 0 ? qbx_class_split_chunks[cls]
+                                                                       : STAGED_ROW_CHUNKS;
+}
+
 // start64: per task range, the first FP64 ket left to these lists (the twin
 // groups take the kets before it); null: all of them
 void task_counts(const qbx_basis* b, int cls, double thr, int mode, double cut, TaskCounts& tc,
@@ -1147,8 +1157,9 @@ void task_counts(const qbx_basis* b, int cls, double thr, int mode, double cut, 
     const PairClass& PY = pair_view(b, cls, 1);
     const int nk = PY.nbk;
     const int nt = nx * nk;
-    const int rc = qbx_class_ws[cls] > 0 ? STAGED_ROW_CHUNKS : ROW_CHUNKS;   // chunks per K-row task
+    const int rc = task_chunks(cls);   // chunks per K-row task
     tc.ntb = nt;
+    tc.rc = rc;
     tc.p64.assign(nt + 1, 0);
     tc.p32.assign(nt + 1, 0);
     tc.k64.assign(std::max(nt, 1), 0);
@@ -1183,6 +1194,7 @@ void upload_tasks(TaskSet& t64, TaskSet& t32, const TaskCounts& tc, hipStream_t 
     t64.cpre.upload(tc.w64, st);
     t64.spre.upload(tc.s64, st);
     t64.nsuper = tc.s64[nt];
+    t64.chunks = tc.rc;
     t64.total = tc.p64[nt];
     t64.nwave = tc.w64[nt];
     t64.nbra = nt;
@@ -1191,6 +1203,7 @@ void upload_tasks(TaskSet& t64, TaskSet& t32, const TaskCounts& tc, hipStream_t 
     t32.cpre.upload(tc.w32, st);
     t32.spre.upload(tc.s32, st);
     t32.nsuper = tc.s32[nt];
+    t32.chunks = tc.rc;
     t32.total = tc.p32[nt];
     t32.nwave = tc.w32[nt];
     t32.nbra = nt;
@@ -1543,6 +1556,66 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
         QBX_HIP(hipGetLastError());
         pdmax = b->d_pdmax.p;
     }
+    // one class's J/K kernels (and, in mixed precision, its FP32 list) on ls
+    auto launch_class = [&](int cls, hipStream_t ls) {
+        if (ctx->timing == 1) QBX_HIP(hipEventRecord(ev[2 * cls], st));
+        if (T) {
+            QBX_HIP(hipEventRecord(T->beg_l[cls], ls));
+            QBX_HIP(hipEventRecord(T->beg_s[cls], st));
+        }
+        EriArgs a = base_args(b, cls);
+        a.pdmax = pdmax;
+        a.scratch_stream = (void*)st;
+        a.nshell = ns;
+        a.dthr = ctx->density_thr;
+        a.D = dD;
+        a.J = b->d_J.p;
+        a.K = b->d_K.p;
+        a.xcd_local = ctx->xcd_local ? 1 : 0;
+        a.jk_stride = NN * nset;
+        a.nset = nset;
+        // twin groups where their LDS rows ([set][member][A and B][N]) fit a block
+        bool twin = b->use_twin[cls] && ctx->jk_rows;
+        if (twin) {
+            const size_t rows = (size_t)nset * b->w64[cls].twmax *
+                                (qbx_ncart(qbx_class_l[cls][0]) + qbx_ncart(qbx_class_l[cls][1])) * b->nbf *
+                                sizeof(double);
+            const size_t boys = (size_t)QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE * sizeof(double);
+            twin = rows <= (size_t)ROW_LDS_MAX && rows + boys <= (size_t)ctx->lds_max;
+        }
+        auto set_tasks = [&](const TaskSet& t, const TwinSet* w) {
+            a.prefix = t.prefix.p;
+            a.koff = t.koff.p;
+            a.cpre = t.cpre.p;
+            // no K-row task lists: the one-wave-per-chunk kernels run
+            a.spre = ctx->jk_rows ? t.spre.p : nullptr;
+            a.nsuper = t.nsuper;
+            a.tchunks = t.chunks;
+            a.total = t.total;
+            a.nwave = t.nwave;
+            a.nbra = t.nbra;
+            a.gpre = w ? w->gpre.p : nullptr;
+            a.gmem = w ? w->gmem.p : nullptr;
+            a.ngroup = w ? w->ngroup : 0;
+            a.ntwtask = w ? w->ntask : 0;
+            a.twmax = w ? w->twmax : 0;
+        };
+        a.lds_max = ctx->lds_max;
+        a.split = ctx->staged_split ? 1 : 0;
+        const TaskSet& t64 = twin ? b->s64[cls] : b->t64[cls];
+        set_tasks(t64, twin ? &b->w64[cls] : nullptr);
+        attach_workspace(b, cls, std::max(t64.nwave, mode == QBX_PREC_MIXED ? b->t32[cls].nwave : 0), a);
+        k_jk64[cls](a, ls);
+        if (mode == QBX_PREC_MIXED) {
+            set_tasks(b->t32[cls], nullptr);
+            k_jk32[cls](a, ls);
+        }
+        if (ctx->timing == 1) QBX_HIP(hipEventRecord(ev[2 * cls + 1], st));
+        if (T) {
+            QBX_HIP(hipEventRecord(T->end_l[cls], ls));
+            QBX_HIP(hipEventRecord(T->end_s[cls], st));
+        }
+    };
     // Without serialized timing, the class kernels run concurrently on the
     // auxiliary streams (staged classes first, then largest work first);
     // they only meet in the J/K atomics.
@@ -1576,66 +1649,7 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
         if (sx != sy) return sx;
         return b->class_cost[x] > b->class_cost[y];
     });
-    for (int oi = 0; oi < QBX_N_CLASSES; ++oi) {
-        const int cls = order[oi];
-        hipStream_t ls = fork ? ctx->aux[oi % kStreams] : st;
-        if (ctx->timing == 1) QBX_HIP(hipEventRecord(ev[2 * cls], st));
-        if (T) {
-            QBX_HIP(hipEventRecord(T->beg_l[cls], ls));
-            QBX_HIP(hipEventRecord(T->beg_s[cls], st));
-        }
-        EriArgs a = base_args(b, cls);
-        a.pdmax = pdmax;
-        a.scratch_stream = (void*)st;
-        a.nshell = ns;
-        a.dthr = ctx->density_thr;
-        a.D = dD;
-        a.J = b->d_J.p;
-        a.K = b->d_K.p;
-        a.xcd_local = ctx->xcd_local ? 1 : 0;
-        a.jk_stride = NN * nset;
-        a.nset = nset;
-        // twin groups where their LDS rows ([set][member][A and B][N]) fit a block
-        bool twin = b->use_twin[cls] && ctx->jk_rows;
-        if (twin) {
-            const size_t rows = (size_t)nset * b->w64[cls].twmax *
-                                (qbx_ncart(qbx_class_l[cls][0]) + qbx_ncart(qbx_class_l[cls][1])) * b->nbf *
-                                sizeof(double);
-            const size_t boys = (size_t)QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE * sizeof(double);
-            twin = rows <= (size_t)ROW_LDS_MAX && rows + boys <= (size_t)ctx->lds_max;
-        }
-        auto set_tasks = [&](const TaskSet& t, const TwinSet* w) {
-            a.prefix = t.prefix.p;
-            a.koff = t.koff.p;
-            a.cpre = t.cpre.p;
-            // no K-row task lists: the one-wave-per-chunk kernels run
-            a.spre = ctx->jk_rows ? t.spre.p : nullptr;
-            a.nsuper = t.nsuper;
-            a.total = t.total;
-            a.nwave = t.nwave;
-            a.nbra = t.nbra;
-            a.gpre = w ? w->gpre.p : nullptr;
-            a.gmem = w ? w->gmem.p : nullptr;
-            a.ngroup = w ? w->ngroup : 0;
-            a.ntwtask = w ? w->ntask : 0;
-            a.twmax = w ? w->twmax : 0;
-        };
-        a.lds_max = ctx->lds_max;
-        a.split = ctx->staged_split ? 1 : 0;
-        const TaskSet& t64 = twin ? b->s64[cls] : b->t64[cls];
-        set_tasks(t64, twin ? &b->w64[cls] : nullptr);
-        attach_workspace(b, cls, std::max(t64.nwave, mode == QBX_PREC_MIXED ? b->t32[cls].nwave : 0), a);
-        k_jk64[cls](a, ls);
-        if (mode == QBX_PREC_MIXED) {
-            set_tasks(b->t32[cls], nullptr);
-            k_jk32[cls](a, ls);
-        }
-        if (ctx->timing == 1) QBX_HIP(hipEventRecord(ev[2 * cls + 1], st));
-        if (T) {
-            QBX_HIP(hipEventRecord(T->end_l[cls], ls));
-            QBX_HIP(hipEventRecord(T->end_s[cls], st));
-        }
-    }
+    for (int oi = 0; oi < QBX_N_CLASSES; ++oi) launch_class(order[oi], fork ? ctx->aux[oi % kStreams] : st);
     if (fork) {
         for (int k = 0; k < kStreams; ++k) {
             QBX_HIP(hipEventRecord(ctx->ev_join[k], ctx->aux[k]));

@@ -916,7 +916,8 @@ __device__ __forceinline__ void staged_groups(std::integer_sequence<int, Gs...>,
                                               const double* __restrict__ sb, R CDx, R CDy, R CDz,
                                               const WsSlot<R>& ws)
 {
-    ((W == 1 || Gs % W == wib ? staged_group<Cls, LT, R, Gs>(BP, bn, KP, kn, sb, CDx, CDy, CDz, ws) : (void)0), ...);
+    ((W == 1 || Cls::grp_wave(Gs) == wib ? staged_group<Cls, LT, R, Gs>(BP, bn, KP, kn, sb, CDx, CDy, CDz, ws)
+                                          : (void)0), ...);
 }
 
 // One wave task of a staged class: bra `bi`, kets chunk*64 .. chunk*64+63 of
@@ -1192,7 +1193,7 @@ void eri_staged_split_kernel(EriArgs a)
 }
 
 // Staged J/K with the bra's K rows in LDS: a persistent grid of blocks, each
-// taking one (bra, up to STAGED_ROW_CHUNKS chunks) task at a time; its waves share
+// taking one (bra, up to a.tchunks chunks) task at a time; its waves share
 // the task's chunks, K goes to the rows, and the rows are flushed to K once
 // per task.
 template <int LA, int LB, int LC, int LD, typename R>
@@ -1206,7 +1207,6 @@ void eri_staged_rows_kernel(EriArgs a)
     constexpr int NROW = NA + NB;
     constexpr int NCD = qbx_ncart(LC) * qbx_ncart(LD);
     constexpr int WSV = Cls::NE * NCD;
-    static_assert(STAGED_ROW_CHUNKS % W == 0, "chunks per task must split evenly over the waves");
     __shared__ double2 sboys[BOYS_HEAD2 + QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE / 2];
     extern __shared__ double krow[];    // [nset][NROW][N]
     const int Ni = a.N;
@@ -1232,8 +1232,9 @@ void eri_staged_rows_kernel(EriArgs a)
         const int tb = lo, bi = tb / a.nbk;
         const int sc = task - spre[tb];
         const int nk = (int)(cst(a.prefix)[tb + 1] - cst(a.prefix)[tb]);
-        for (int r = 0; r < STAGED_ROW_CHUNKS / W; ++r) {
-            const int chunk = (sc * (STAGED_ROW_CHUNKS / W) + r) * W + wib;
+        // wave wib takes chunks wib, wib + W, ... of the task's a.tchunks
+        for (int j = wib; j < a.tchunks; j += W) {
+            const int chunk = sc * a.tchunks + j;
             if (chunk * 64 >= nk) break;
             staged_chunk<LA, LB, LC, LD, R, MODE_JK>(a, tb, chunk, lane, ws, sb, kout);
         }
@@ -1247,7 +1248,7 @@ void eri_staged_rows_kernel(EriArgs a)
 // Staged J/K with the bra's K rows in LDS and each 64-ket chunk shared by
 // the Cls::SPLIT waves of a block (groups, then digest rows, spread over the
 // waves; one workspace slot per block).  A task is one (bra, up to
-// STAGED_ROW_CHUNKS chunks) range, its chunks taken one after another by the
+// a.tchunks chunks) range, its chunks taken one after another by the
 // whole block, so no wave idles when a bra has fewer chunks than the block
 // has waves; the rows are flushed to K once per task.
 template <int LA, int LB, int LC, int LD, typename R>
@@ -1286,8 +1287,8 @@ void eri_staged_split_rows_kernel(EriArgs a)
         const int tb = lo, bi = tb / a.nbk;
         const int sc = task - spre[tb];
         const int nk = (int)(cst(a.prefix)[tb + 1] - cst(a.prefix)[tb]);
-        for (int r = 0; r < STAGED_ROW_CHUNKS; ++r) {
-            const int chunk = sc * STAGED_ROW_CHUNKS + r;
+        for (int r = 0; r < a.tchunks; ++r) {
+            const int chunk = sc * a.tchunks + r;
             if (chunk * 64 >= nk) break;
             staged_chunk<LA, LB, LC, LD, R, MODE_JK, KToRows<NROW>, W>(a, tb, chunk, lane, ws, sb, kout, wib);
             __syncthreads();    // the slot is rewritten by the next chunk

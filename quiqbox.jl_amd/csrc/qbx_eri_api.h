@@ -53,8 +53,9 @@ struct EriArgs {
     const int64_t* prefix;    // [nbra+1] prefix over bras of the screened ket counts
     const int* koff;          // [nbra] first ket of each bra
     const int* cpre;          // [nbra+1] prefix over bras of ceil(kets / 64): waves
-    const int* spre;          // [nbra+1] prefix over bras of ceil(waves / ROW_CHUNKS): K-row blocks
+    const int* spre;          // [nbra+1] prefix over bras of ceil(waves / tchunks): K-row blocks
     int nsuper;               // number of K-row blocks
+    int tchunks;              // chunks per K-row task of the staged kernels (spre)
     const int* gpre;          // [ngroup+1] twin groups: prefix of tasks (ROW_CHUNKS chunks of the union range)
     const int* gmem;          // [ngroup][TWIN_MAX][3]: bra, first ket, end of kets (bra < 0: no member)
     int ngroup;               // twin groups of this class (0: none)
