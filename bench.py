@@ -301,6 +301,44 @@ def learned_cho_section():
     return out
 
 
+def scf_trace(atoms, seeds, steps, with_d=True):
+    """RHF on the Fock-build workload's random C/H/O clusters (s, p and d
+    shells; s and p only with with_d=False) from the core guess with three of Quiqbox's strategies, at most
+    `steps` SCF steps each: per seed and strategy, convergence, the number of
+    steps, the energies of the last ten steps, and the HOMO-LUMO gap of the
+    last Fock matrix."""
+    import quiqbox_amd as qb
+    from quiqbox_amd.systems import random_cluster
+    out = []
+    for seed in seeds:
+        nuc, bs = random_cluster(atoms, seed, with_d=with_d)
+        gb = qb.GPUBasis(bs)
+        S = gb.overlap()
+        w, U = np.linalg.eigh(S)
+        X = U / np.sqrt(w)
+        nocc = int(round(sum(nuc.charges))) // 2
+        for name, strat in (("ADIIS+DIIS", qb.SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-8))),
+                            ("DIIS", qb.SCFconfig(("DIIS",), (1e-8,))),
+                            ("EDIIS+DIIS", qb.SCFconfig(("EDIIS", "DIIS"), (1e-4, 1e-8)))):
+            t0 = time.perf_counter()
+            res = qb.runHartreeFock(nuc, gb, qb.HFconfig(initial="CoreH", strategy=strat, maxStep=steps),
+                                    printInfo=False)
+            t = time.perf_counter() - t0
+            E = [float(e) for e in res.memory[0].shared.Etots]
+            eps = np.linalg.eigvalsh(X.T @ np.asarray(res.fock[0]) @ X)
+            gap = float(eps[nocc] - eps[nocc - 1])
+            tail = E[-10:]
+            rec = {"atoms": atoms, "seed": seed, "functions": gb.n, "strategy": name, "converged": bool(res.converged),
+                   "steps": len(E) - 1, "wall_s": t, "last_energies": tail, "last10_spread": max(tail) - min(tail),
+                   "homo_lumo_gap": gap}
+            out.append(rec)
+            log("scf trace: %d atoms seed %d (%d functions) %s: %s after %d steps (%.1f s); last ten energies "
+                "%s; spread %.2e Eh; HOMO-LUMO gap %.4f Eh" % (
+                    atoms, seed, gb.n, name, "converged" if res.converged else "not converged", len(E) - 1, t,
+                    " ".join("%.6f" % e for e in tail), rec["last10_spread"], gap))
+    return out
+
+
 def gradient_section(steps):
     """Basis-parameter gradient on the 32-atom H cluster with floating
     4-primitive s and p shells (256 exponents, 256 coefficients, 96 centre
@@ -436,6 +474,10 @@ def main():
     ap.add_argument("--learned-systems", type=int, default=40,
                     help="training systems of the learned-guess section (0: skip it)")
     ap.add_argument("--learned-epochs", type=int, default=300)
+    ap.add_argument("--scf-trace", default=None, metavar="SEEDS",
+                    help="only run RHF on the workload's clusters of these seeds (comma list) and print the traces")
+    ap.add_argument("--scf-steps", type=int, default=100)
+    ap.add_argument("--scf-sp", action="store_true", help="--scf-trace on s/p clusters (no d shells)")
     ap.add_argument("--no-cho", action="store_true",
                     help="skip the packaged C/H/O network's evaluation on unseen C/H/O clusters")
     ap.add_argument("--grad-steps", type=int, default=100,
@@ -454,6 +496,11 @@ def main():
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.scf_trace is not None:
+        recs = scf_trace(args.atoms, [int(x) for x in args.scf_trace.split(",") if x], args.scf_steps,
+                         with_d=not args.scf_sp)
+        print(json.dumps({"scf_trace": recs}), flush=True)
+        return
 
     import quiqbox_amd as qb
     from quiqbox_amd.systems import random_cluster
