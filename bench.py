@@ -168,6 +168,19 @@ def host_cpu():
 _NATIVE_ORACLE = []
 
 
+def native_oracle():
+    """The oracle module, built once per process for this host (-march=native,
+    OpenMP) in a temporary directory: the CPU baseline's timing build."""
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    if not _NATIVE_ORACLE:
+        d = tempfile.mkdtemp(prefix="qbx_oracle_")
+        oracle.use_native_build(d)
+        _NATIVE_ORACLE.append(d)
+    return oracle
+
+
 def cpu_baseline(flat, n_quartets_total, seconds=12.0, threads=None):
     """Time the oracle (reference algorithm restated in C, compiled here for
     this host with -march=native and OpenMP) on random unique contracted
@@ -175,18 +188,12 @@ def cpu_baseline(flat, n_quartets_total, seconds=12.0, threads=None):
     affinity set, capped by OMP_NUM_THREADS where the machine sets it);
     returns quartets/s and the extrapolated time of one full screened Fock
     build's ERIs."""
-    import tempfile
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
+    oracle = native_oracle()
     cpu = host_cpu()
     if threads is None:
         threads = cpu["affinity_cpus"]
         if cpu["omp_num_threads"]:
             threads = min(threads, cpu["omp_num_threads"])
-    if not _NATIVE_ORACLE:
-        d = tempfile.mkdtemp(prefix="qbx_oracle_")
-        oracle.use_native_build(d)
-        _NATIVE_ORACLE.append(d)
     ob = oracle.Basis(flat.as_dict())
     n = flat.n_bf
     rng = np.random.default_rng(0)
@@ -298,6 +305,58 @@ def learned_cho_section():
         ", ".join("%s converged %d/%d" % (g, out[g]["converged"], out["clusters"]) for g in guesses),
         out["all_converged"], ", ".join("%s %d builds %.2f s" % (g, out[g]["builds_all_converged"],
                                                                 out[g]["wall_s_all_converged"]) for g in guesses)))
+    return out
+
+
+def reference_benchmark_section(no_cpu):
+    """The reference's own integral benchmarks (benchmark/benchmarks.jl:29-30,
+    46, 70-72, 203-212): N2 with the nuclei at (0, 0, -1) and (0, 0, 1) bohr in
+    cc-pVTZ (70 Cartesian functions, f shells included).  elecRepulsions (the
+    dense ERI tensor, returned to the host) and nucAttractions through the
+    public API on the GPU, each the median of five calls after one warm-up
+    call; beside them the oracle's dense tensor on the host's threads (the CPU
+    baseline) and the largest difference between the two tensors."""
+    import quiqbox_amd as qb
+    coords = [(0.0, 0.0, -1.0), (0.0, 0.0, 1.0)]
+    bs = []
+    for sym, c in zip(("N", "N"), coords):
+        bs += qb.genGaussTypeOrbSeq(c, sym, "cc-pVTZ")
+
+    def med(f):
+        f()
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            r = f()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), r
+
+    t_eri, E = med(lambda: qb.elecRepulsions(bs))
+    t_nuc, _ = med(lambda: qb.nucAttractions(["N", "N"], coords, bs))
+    n = E.shape[0]
+    out = {"system": "N2 cc-pVTZ, nuclei at (0, 0, -1) and (0, 0, 1) bohr", "basis_functions": n,
+           "elecRepulsions_s": t_eri, "nucAttractions_s": t_nuc,
+           "unique_quartets": n * (n + 1) // 2 * (n * (n + 1) // 2 + 1) // 2}
+    log("reference benchmark (N2 cc-pVTZ, %d functions): elecRepulsions %.1f ms (dense %d^4 tensor to the host), "
+        "nucAttractions %.2f ms" % (n, 1e3 * t_eri, n, 1e3 * t_nuc))
+    if not no_cpu:
+        oracle = native_oracle()
+        cpu = host_cpu()
+        threads = cpu["affinity_cpus"]
+        if cpu["omp_num_threads"]:
+            threads = min(threads, cpu["omp_num_threads"])
+        ob = oracle.Basis(qb.flatten_basis(bs).as_dict())
+        ii, jj = np.tril_indices(n)
+        P, Q = np.tril_indices(len(ii))
+        q = np.stack([ii[P], jj[P], ii[Q], jj[Q]], axis=1).astype(np.int32)
+        t0 = time.perf_counter()
+        v = ob.eri_list(q, threads=threads)
+        t_cpu = time.perf_counter() - t0
+        dmax = float(np.max(np.abs(E[q[:, 0], q[:, 1], q[:, 2], q[:, 3]] - v)))
+        out["cpu_baseline"] = {"oracle_unique_quartets_s": t_cpu, "threads": threads, "kind": "port",
+                               "max_abs_difference": dmax}
+        log("  oracle: the %d unique quartets in %.2f s on %d threads (GPU %.0fx faster); max |GPU - oracle| %.1e"
+            % (len(q), t_cpu, threads, t_cpu / t_eri, dmax))
     return out
 
 
@@ -478,6 +537,8 @@ def main():
                     help="only run RHF on the workload's clusters of these seeds (comma list) and print the traces")
     ap.add_argument("--scf-steps", type=int, default=100)
     ap.add_argument("--scf-sp", action="store_true", help="--scf-trace on s/p clusters (no d shells)")
+    ap.add_argument("--no-refbench", action="store_true",
+                    help="skip the reference's N2 cc-pVTZ integral benchmarks")
     ap.add_argument("--no-cho", action="store_true",
                     help="skip the packaged C/H/O network's evaluation on unseen C/H/O clusters")
     ap.add_argument("--grad-steps", type=int, default=100,
@@ -619,6 +680,8 @@ def main():
             out["learned_guess_cho"] = learned_cho_section()
         if args.grad_steps > 0:
             out["gradient"] = gradient_section(args.grad_steps)
+    if not args.no_refbench:
+        out["reference_benchmark"] = reference_benchmark_section(args.no_cpu)
     if args.mo_size > 0:
         out["mo_transform"] = mo_section(args.mo_size)
     print(json.dumps(out), flush=True)

@@ -18,8 +18,9 @@
 //
 // Work mapping: one wavefront per shell quartet, its lanes over the
 // quartet's Cartesian component quartets (strides of 64), each lane summing
-// its component over all primitive quartets.  Per-lane recurrence tables of
-// (NM+1)^2 doubles live in scratch.
+// its component over all primitive quartets; a quartet with fewer than 32
+// components gives each component several lanes, which split its primitive
+// quartets.  Per-lane recurrence tables of (NM+1)^2 doubles live in scratch.
 #include "qbx_eri.h"
 
 namespace qbx {
@@ -227,39 +228,68 @@ __global__ __launch_bounds__(256) void eri_generic_kernel(GenArgs a)
     ea.xcd_local = a.xcd_local;
     ea.jk_stride = a.jk_stride;
     const JKSink sink(ea);
-    for (int cidx = lane; cidx < ncomp; cidx += 64) {
-        int ia, ib, ic, id;
-        if (MODE == MODE_SCHWARZ) {
-            ia = cidx / NB;
-            ib = cidx - ia * NB;
-            ic = ia;
-            id = ib;
-        } else {
-            int r = cidx;
-            id = r % ND;
-            r /= ND;
-            ic = r % NC;
-            r /= NC;
-            ib = r % NB;
-            ia = r / NB;
-        }
-        const int fa = SA->bf[ia], fb = SB->bf[ib], fc = SC->bf[ic], fd = SD->bf[id];
-        if (fa < 0 || fb < 0 || fc < 0 || fd < 0) continue;
-        int ga[3], gb[3], gc[3], gd[3];
-        qbx_cart(SA->L, ia, ga);
-        qbx_cart(SB->L, ib, gb);
-        qbx_cart(SC->L, ic, gc);
-        qbx_cart(SD->L, id, gd);
-        double acc = 0.0;
-        for (int u = 0; u < bn; ++u) {
-            const PrimGeom X = load_geom(a, a.pp[bo + u], bo + u, sA, sB, AB);
-            for (int v = 0; v < kn; ++v) {
-                const PrimGeom Y = load_geom(a, a.pp[ko + v], ko + v, sC, sD, CD);
-                int la[3] = {ga[0], ga[1], ga[2]}, lb[3] = {gb[0], gb[1], gb[2]};
-                int lc[3] = {gc[0], gc[1], gc[2]}, ld[3] = {gd[0], gd[1], gd[2]};
-                acc += prim_quartet<NM>(X, Y, la, lb, lc, ld, a.boys, a.bexp);
+    // Lanes per component quartet: a quartet with fewer than 32 Cartesian
+    // component quartets (an f shell with s shells, say) would leave most of
+    // the wave idle, so lpc lanes (a power of two, lpc * ncomp <= 64) share
+    // one component's primitive quartets and sum them with shuffles.
+    int lpc = 1;
+    while (2 * lpc * ncomp <= 64) lpc *= 2;
+    const int sub = lane & (lpc - 1);
+    const int npq = bn * kn;
+    for (int c0 = 0; c0 < ncomp; c0 += 64 / lpc) {
+        const int cidx = c0 + lane / lpc;
+        int ia = 0, ib = 0, ic = 0, id = 0;
+        bool live = cidx < ncomp;
+        if (live) {
+            if (MODE == MODE_SCHWARZ) {
+                ia = cidx / NB;
+                ib = cidx - ia * NB;
+                ic = ia;
+                id = ib;
+            } else {
+                int r = cidx;
+                id = r % ND;
+                r /= ND;
+                ic = r % NC;
+                r /= NC;
+                ib = r % NB;
+                ia = r / NB;
             }
         }
+        const int fa = live ? SA->bf[ia] : -1, fb = live ? SB->bf[ib] : -1;
+        const int fc = live ? SC->bf[ic] : -1, fd = live ? SD->bf[id] : -1;
+        live = live && fa >= 0 && fb >= 0 && fc >= 0 && fd >= 0;
+        double acc = 0.0;
+        if (live) {
+            int ga[3], gb[3], gc[3], gd[3];
+            qbx_cart(SA->L, ia, ga);
+            qbx_cart(SB->L, ib, gb);
+            qbx_cart(SC->L, ic, gc);
+            qbx_cart(SD->L, id, gd);
+            if (lpc == 1) {
+                for (int u = 0; u < bn; ++u) {
+                    const PrimGeom X = load_geom(a, a.pp[bo + u], bo + u, sA, sB, AB);
+                    for (int v = 0; v < kn; ++v) {
+                        const PrimGeom Y = load_geom(a, a.pp[ko + v], ko + v, sC, sD, CD);
+                        int la[3] = {ga[0], ga[1], ga[2]}, lb[3] = {gb[0], gb[1], gb[2]};
+                        int lc[3] = {gc[0], gc[1], gc[2]}, ld[3] = {gd[0], gd[1], gd[2]};
+                        acc += prim_quartet<NM>(X, Y, la, lb, lc, ld, a.boys, a.bexp);
+                    }
+                }
+            } else {
+                for (int t = sub; t < npq; t += lpc) {
+                    const int u = t / kn, v = t - u * kn;
+                    const PrimGeom X = load_geom(a, a.pp[bo + u], bo + u, sA, sB, AB);
+                    const PrimGeom Y = load_geom(a, a.pp[ko + v], ko + v, sC, sD, CD);
+                    int la[3] = {ga[0], ga[1], ga[2]}, lb[3] = {gb[0], gb[1], gb[2]};
+                    int lc[3] = {gc[0], gc[1], gc[2]}, ld[3] = {gd[0], gd[1], gd[2]};
+                    acc += prim_quartet<NM>(X, Y, la, lb, lc, ld, a.boys, a.bexp);
+                }
+            }
+        }
+        // every lane of the wave takes part in the shuffles (lpc is uniform)
+        for (int off = lpc >> 1; off > 0; off >>= 1) acc += __shfl_down(acc, off, lpc);
+        if (!live || sub != 0) continue;
         const double val = acc * (SA->sc[ia] * SB->sc[ib]) * (SC->sc[ic] * SD->sc[id]);
         if (MODE == MODE_SCHWARZ) {
             atomic_max_nonneg(a.qbits + bi, fabs(val));
