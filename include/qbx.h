@@ -232,6 +232,15 @@ int  qbx_rhf_gradient(qbx_ctx* ctx, qbx_basis* basis, const double* Dt, const do
                       const double* Z, const double* R, double schwarz_thr, double* g_xpn, double* g_lin,
                       double* g_cen);
 
+/* The same gradient for a UHF state (the reference's UOHartreeFock,
+   HartreeFock.jl:17-24): Dt = D_alpha + D_beta, the spin densities
+   D_s = C_s,occ C_s,occ^T, and W = sum_s C_s,occ diag(eps_s,occ) C_s,occ^T;
+   the two-electron term is sum (1/2 Dt_mn Dt_ls - 1/2 sum_s D_s,ml D_s,ns)
+   (mn|ls).  With D_alpha = D_beta = Dt/2 it equals qbx_rhf_gradient. */
+int  qbx_uhf_gradient(qbx_ctx* ctx, qbx_basis* basis, const double* Dt, const double* D_alpha,
+                      const double* D_beta, const double* W, int32_t n_nuc, const double* Z, const double* R,
+                      double schwarz_thr, double* g_xpn, double* g_lin, double* g_cen);
+
 /* ---- Density from the Fock matrix without diagonalization ----------------
    The reference forms each SCF step's density from eigen(X' F X)
    (solveFockMatrix/getC/getD, HartreeFock.jl:46-59, 296-302); this entry point

@@ -2687,12 +2687,21 @@ int qbx_basis_shells(const qbx_basis* b, int32_t* n_shell, int32_t* n_radial, in
     });
 }
 
-int qbx_rhf_gradient(qbx_ctx* ctx, qbx_basis* b, const double* Dt, const double* W, int32_t n_nuc,
-                     const double* Z, const double* R, double thr, double* g_xpn, double* g_lin, double* g_cen)
+}  // extern "C"
+
+namespace {
+
+// RHF (Da = Db = null) or UHF energy gradient; see qbx_rhf_gradient and
+// qbx_uhf_gradient in qbx.h.
+void hf_gradient(qbx_ctx* ctx, qbx_basis* b, const double* Dt, const double* Da, const double* Db, const double* W,
+                 int32_t n_nuc, const double* Z, const double* R, double thr, double* g_xpn, double* g_lin,
+                 double* g_cen)
 {
-    return guarded([&] {
+    {
         check_ctx_basis(ctx, b);
         if (!Dt || !W || !g_xpn || !g_lin || !g_cen) throw QbxError(QBX_EINVAL, "null density or output pointer");
+        if ((Da == nullptr) != (Db == nullptr)) throw QbxError(QBX_EINVAL, "UHF needs both spin densities");
+        const bool uhf = Da != nullptr;
         if (n_nuc <= 0 || !Z || !R) throw QbxError(QBX_EINVAL, "the gradient needs at least one nucleus");
         if (!(thr >= 0.0)) throw QbxError(QBX_EINVAL, "Schwarz threshold must be >= 0");
         if (b->maxL > 1) throw QbxError(QBX_EUNSUP, "the gradient kernels cover s and p shells");
@@ -2707,7 +2716,15 @@ int qbx_rhf_gradient(qbx_ctx* ctx, qbx_basis* b, const double* Dt, const double*
         const size_t NN = (size_t)b->nbf * b->nbf;
         const int ns = (int)b->shells.size(), nr = (int)b->pexp.size();
         DevBuf<double> dD, dW, dZ, dR, gx, gl, gc;
-        dD.upload(std::vector<double>(Dt, Dt + NN), st);
+        {
+            // [D] for RHF, [D, D_alpha, D_beta] for UHF
+            std::vector<double> h(Dt, Dt + NN);
+            if (uhf) {
+                h.insert(h.end(), Da, Da + NN);
+                h.insert(h.end(), Db, Db + NN);
+            }
+            dD.upload(h, st);
+        }
         dW.upload(std::vector<double>(W, W + NN), st);
         dZ.upload(std::vector<double>(Z, Z + n_nuc), st);
         dR.upload(std::vector<double>(R, R + 3 * n_nuc), st);
@@ -2755,6 +2772,7 @@ int qbx_rhf_gradient(qbx_ctx* ctx, qbx_basis* b, const double* Dt, const double*
             const int lt = qbx_class_l[cls][0] + qbx_class_l[cls][1] + qbx_class_l[cls][2] + qbx_class_l[cls][3];
             a.boysl = ctx->boysl.p + (size_t)(lt + 1) * QBX_BOYSL_ROWS * QBX_BOYSL_STRIDE;
             a.D = dD.p;
+            a.nset = uhf ? 2 : 1;
             a.prefix = t64.prefix.p;
             a.koff = t64.koff.p;
             a.cpre = t64.cpre.p;
@@ -2772,6 +2790,26 @@ int qbx_rhf_gradient(qbx_ctx* ctx, qbx_basis* b, const double* Dt, const double*
         QBX_HIP(hipMemcpyAsync(g_lin, gl.p, nr * sizeof(double), hipMemcpyDeviceToHost, st));
         QBX_HIP(hipMemcpyAsync(g_cen, gc.p, 3 * (size_t)ns * sizeof(double), hipMemcpyDeviceToHost, st));
         QBX_HIP(hipStreamSynchronize(st));
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int qbx_rhf_gradient(qbx_ctx* ctx, qbx_basis* b, const double* Dt, const double* W, int32_t n_nuc,
+                     const double* Z, const double* R, double thr, double* g_xpn, double* g_lin, double* g_cen)
+{
+    return guarded([&] { hf_gradient(ctx, b, Dt, nullptr, nullptr, W, n_nuc, Z, R, thr, g_xpn, g_lin, g_cen); });
+}
+
+int qbx_uhf_gradient(qbx_ctx* ctx, qbx_basis* b, const double* Dt, const double* Da, const double* Db,
+                     const double* W, int32_t n_nuc, const double* Z, const double* R, double thr, double* g_xpn,
+                     double* g_lin, double* g_cen)
+{
+    return guarded([&] {
+        if (!Da || !Db) throw QbxError(QBX_EINVAL, "null spin density");
+        hf_gradient(ctx, b, Dt, Da, Db, W, n_nuc, Z, R, thr, g_xpn, g_lin, g_cen);
     });
 }
 

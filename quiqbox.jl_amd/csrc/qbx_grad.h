@@ -8,6 +8,10 @@
 //      of the unique quartet, s the component scales), so that the sum of
 //      G (ab|cd) over all unique quartets is the two-electron energy
 //      sum (1/2 D_mn D_ls - 1/4 D_ml D_ns) (mn|ls) over all index quartets;
+//      for UHF (nset = 2: D holds D, D_alpha, D_beta) the exchange part is
+//      2 sum_s (D^s_ac D^s_bd + D^s_ad D^s_bc), i.e. the energy
+//      sum (1/2 D_mn D_ls - 1/2 sum_s D^s_ml D^s_ns) (mn|ls), which is the
+//      RHF form when D_alpha = D_beta = D/2;
 //   2. runs the forward primitive loop (contracted [e0|f0], as the Fock build);
 //   3. transposes the horizontal recurrence (EriAdj::hrr_adj): seeds on the
 //      contracted [e0|f0] and the derivatives with respect to A-B and C-D;
@@ -168,8 +172,17 @@ __global__ __launch_bounds__(256) void eri_grad_kernel(EriArgs a)
                     const int m = max(SA->bf[ia], 0), n = max(SB->bf[ib], 0);
                     const int l = max(SC->bf[ic], 0), s = max(SD->bf[id], 0);
                     const double sc = SA->sc[ia] * SB->sc[ib] * SC->sc[ic] * SD->sc[id];
-                    const double gam = 4.0 * D[m * N + n] * D[l * N + s] - D[m * N + l] * D[n * N + s] -
-                                       D[m * N + s] * D[n * N + l];
+                    double gam;
+                    if (a.nset == 2) {
+                        const double* Da = D + N * N;
+                        const double* Db = D + 2 * N * N;
+                        gam = 4.0 * D[m * N + n] * D[l * N + s] -
+                              2.0 * (Da[m * N + l] * Da[n * N + s] + Db[m * N + l] * Db[n * N + s] +
+                                     Da[m * N + s] * Da[n * N + l] + Db[m * N + s] * Db[n * N + l]);
+                    } else {
+                        gam = 4.0 * D[m * N + n] * D[l * N + s] - D[m * N + l] * D[n * N + s] -
+                              D[m * N + s] * D[n * N + l];
+                    }
                     G[((ia * NB + ib) * NC + ic) * ND + id] = f * sc * gam;
                 }
 

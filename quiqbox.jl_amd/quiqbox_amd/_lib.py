@@ -55,6 +55,7 @@ SIGNATURES = {
     "qbx_prim_eri": (_i32, [_vp, _i32, _pd, _pd, _pi, _pd]),
     "qbx_basis_shells": (_i32, [_vp, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd]),
     "qbx_rhf_gradient": (_i32, [_vp, _vp, _pd, _pd, _i32, _pd, _pd, _d, _pd, _pd, _pd]),
+    "qbx_uhf_gradient": (_i32, [_vp, _vp, _pd, _pd, _pd, _pd, _i32, _pd, _pd, _d, _pd, _pd, _pd]),
     "qbx_density_purify": (_i32, [_vp, _i32, _i32, _pd, _pd, _pd, _i32, _pi, _pd]),
     "qbx_fock_solve": (_i32, [_vp, _i32, _i32, _pd, _pd, _pd, _pd, _pd]),
     "qbx_density_purify_noalloc": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp, _i32, _pi, _pd, _vp]),

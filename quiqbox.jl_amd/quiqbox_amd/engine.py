@@ -356,6 +356,22 @@ class GPUBasis:
                                                     _lib.dp(R), float(thr), _lib.dp(gx), _lib.dp(gl), _lib.dp(gc)))
         return gx, gl, gc.reshape(ns, 3), sh
 
+    def uhf_gradient(self, Dt, Da, Db, W, Z, R, thr=1e-12):
+        """qbx_uhf_gradient: as rhf_gradient for a UHF state (Dt = Da + Db,
+        the spin densities Da, Db, W summed over both spins)."""
+        if self.T is not None:
+            raise ValueError("the gradient needs a standard (single-centre) basis")
+        Dt, Da, Db, W = _lib.f64(Dt), _lib.f64(Da), _lib.f64(Db), _lib.f64(W)
+        Z, R = _lib.f64(Z), _lib.f64(R).reshape(-1)
+        sh = self.shells()
+        nr, ns = len(sh["xpn"]), len(sh["L"])
+        gx, gl, gc = np.zeros(nr), np.zeros(nr), np.zeros(3 * ns)
+        with self.ctx.lock:
+            _lib.check(_lib.load().qbx_uhf_gradient(self.ctx.h, self.h, _lib.dp(Dt), _lib.dp(Da), _lib.dp(Db),
+                                                    _lib.dp(W), len(Z), _lib.dp(Z), _lib.dp(R), float(thr),
+                                                    _lib.dp(gx), _lib.dp(gl), _lib.dp(gc)))
+        return gx, gl, gc.reshape(ns, 3), sh
+
     def jk_stats(self, thr=1e-12, cut=0.0):
         n = _lib.load().qbx_n_classes()
         arrs = [np.zeros(n, dtype=np.int64) for _ in range(4)]

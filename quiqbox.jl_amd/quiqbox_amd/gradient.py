@@ -1,4 +1,4 @@
-"""RHF energy gradient with respect to the basis parameters, and basis-set
+"""RHF and UHF energy gradients with respect to the basis parameters, and basis-set
 optimization built on it.
 
 Quiqbox v0.6.3 keeps the parameters of a basis in boxes (`Parameters.jl:14-43`,
@@ -123,26 +123,10 @@ def _lnS_derivatives(L, ijk, x, c, h=1e-6):
     return dc, dx
 
 
-def rhf_gradient(nuc, pb, config=None, schwarz=1e-12, res=None, gpu_basis=None):
-    """E_RHF (electronic + nuclear) and dE/dtheta of the ParamBasis `pb`.
-    Runs the GPU SCF unless a converged result `res` of the same basis is given."""
-    gb = gpu_basis or GPUBasis(pb.orbitals())
-    if res is None:
-        cfg = config or HFconfig(initial="CoreH", strategy=SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-11)), maxStep=200)
-        res = runHartreeFock(nuc, gb, cfg, printInfo=False, schwarzThreshold=schwarz)
-    if len(res.coeff) != 1:
-        raise ValueError("rhf_gradient needs a restricted closed-shell result")
-    C = res.coeff[0]
-    eps = np.asarray(res.occu[0][0])
-    nocc = int(sum(1 for o in res.occu[0][1] if o[0]))
-    Co = C[:, :nocc]
-    Dt = 2.0 * Co @ Co.T
-    W = 2.0 * (Co * eps[:nocc]) @ Co.T
-    gx, gl, gc, sh = gb.rhf_gradient(Dt, W, nuc.charges, nuc.coord_array, schwarz)
-    F = res.fock[0]
-    S = gb.overlap()
-    t = 2.0 * np.einsum("ij,ji->i", Dt, F) - 2.0 * np.einsum("ij,ji->i", W, S)
-
+def _chain(pb, gx, gl, gc, sh, t):
+    """dE/dtheta of the ParamBasis from the kernels' per-primitive and
+    per-shell sums (weights held fixed) and t_mu, the derivative with respect
+    to scaling function mu, through the normalization (module docstring)."""
     # kernel shell of each user shell: match the first component's function
     first_bf = {}
     for k in range(len(sh["L"])):
@@ -171,7 +155,65 @@ def rhf_gradient(nuc, pb, config=None, schwarz=1e-12, res=None, gpu_basis=None):
         g_c.append(dEdc)
         gR[s] += gc[k]
         f0 += len(subshellXYZs(L))
-    grad = np.concatenate([np.concatenate(g_x), np.concatenate(g_c), gR.ravel()])
+    return np.concatenate([np.concatenate(g_x), np.concatenate(g_c), gR.ravel()])
+
+
+def _occupied(res, sector):
+    """Occupied coefficients and orbital energies of one spin sector."""
+    C = res.coeff[sector]
+    eps = np.asarray(res.occu[sector][0])
+    occ = np.array([bool(o[0] or o[1]) for o in res.occu[sector][1]])
+    return C[:, occ], eps[occ]
+
+
+def rhf_gradient(nuc, pb, config=None, schwarz=1e-12, res=None, gpu_basis=None):
+    """E_RHF (electronic + nuclear) and dE/dtheta of the ParamBasis `pb`.
+    Runs the GPU SCF unless a converged result `res` of the same basis is given."""
+    gb = gpu_basis or GPUBasis(pb.orbitals())
+    if res is None:
+        cfg = config or HFconfig(initial="CoreH", strategy=SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-11)), maxStep=200)
+        res = runHartreeFock(nuc, gb, cfg, printInfo=False, schwarzThreshold=schwarz)
+    if len(res.coeff) != 1:
+        raise ValueError("rhf_gradient needs a restricted closed-shell result")
+    C = res.coeff[0]
+    eps = np.asarray(res.occu[0][0])
+    nocc = int(sum(1 for o in res.occu[0][1] if o[0]))
+    Co = C[:, :nocc]
+    Dt = 2.0 * Co @ Co.T
+    W = 2.0 * (Co * eps[:nocc]) @ Co.T
+    gx, gl, gc, sh = gb.rhf_gradient(Dt, W, nuc.charges, nuc.coord_array, schwarz)
+    F = res.fock[0]
+    S = gb.overlap()
+    t = 2.0 * np.einsum("ij,ji->i", Dt, F) - 2.0 * np.einsum("ij,ji->i", W, S)
+    grad = _chain(pb, gx, gl, gc, sh, t)
+    E = res.energy[0] + res.energy[1]
+    return E, grad, res
+
+
+def uhf_gradient(nuc, pb, config=None, schwarz=1e-12, res=None, gpu_basis=None):
+    """E_UHF (electronic + nuclear) and dE/dtheta of the ParamBasis `pb`
+    (qbx_uhf_gradient; the chain rule as rhf_gradient with
+    t_mu = 2 sum_s (D_s F_s)_mumu - 2 (W S)_mumu).  Runs the GPU UHF SCF
+    (the reference's UOHartreeFock) unless a converged UHF result `res` of
+    the same basis is given."""
+    from .hartree_fock import UOHartreeFock
+    gb = gpu_basis or GPUBasis(pb.orbitals())
+    if res is None:
+        cfg = config or HFconfig(UOHartreeFock(), initial="CoreH",
+                                 strategy=SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-11)), maxStep=200)
+        res = runHartreeFock(nuc, gb, cfg, printInfo=False, schwarzThreshold=schwarz)
+    if len(res.coeff) != 2:
+        raise ValueError("uhf_gradient needs an unrestricted result")
+    Ds, W = [], 0.0
+    for sector in (0, 1):
+        Co, eo = _occupied(res, sector)
+        Ds.append(Co @ Co.T)
+        W = W + (Co * eo) @ Co.T
+    Dt = Ds[0] + Ds[1]
+    gx, gl, gc, sh = gb.uhf_gradient(Dt, Ds[0], Ds[1], W, nuc.charges, nuc.coord_array, schwarz)
+    S = gb.overlap()
+    t = 2.0 * sum(np.einsum("ij,ji->i", Ds[k], res.fock[k]) for k in (0, 1)) - 2.0 * np.einsum("ij,ji->i", W, S)
+    grad = _chain(pb, gx, gl, gc, sh, t)
     E = res.energy[0] + res.energy[1]
     return E, grad, res
 

@@ -10,7 +10,8 @@ import pytest
 import quiqbox_amd as qb
 from helpers import OracleEngine
 from quiqbox_amd._lib import QbxError
-from quiqbox_amd.gradient import ParamBasis, rhf_gradient
+from quiqbox_amd.gradient import ParamBasis, rhf_gradient, uhf_gradient
+from quiqbox_amd.hartree_fock import UOHartreeFock
 from quiqbox_amd.systems import h_cluster_sp
 
 pytestmark = pytest.mark.gpu
@@ -25,6 +26,10 @@ def _say(msg):
 # 32 atoms: |dE| reaches 1e-14 at a 1e-12 threshold, but the secondary density and
 # commutator criteria (x1000) are not met; 1e-10 converges
 LOOSE = qb.HFconfig(initial="CoreH", strategy=qb.SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-10)), maxStep=300)
+
+
+TIGHT_U = qb.HFconfig(UOHartreeFock(), initial="CoreH", strategy=qb.SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-12)),
+                      maxStep=300)
 
 
 def _energy(nuc, pb, engine="gpu", cfg=TIGHT):
@@ -94,3 +99,46 @@ def test_unsupported_inputs_raise():
     gd = qb.GPUBasis(d_shell.orbitals())
     with pytest.raises(QbxError):
         gd.rhf_gradient(np.eye(gd.n), np.eye(gd.n), [1.0], np.zeros((1, 3)))
+
+
+def test_uhf_gradient_of_a_closed_shell_state_equals_rhf():
+    """With D_alpha = D_beta = D/2 the UHF seeds and chain rule reduce to the
+    RHF ones: a closed-shell cluster whose UHF converges to the RHF state."""
+    nuc, _ = h_cluster_sp(4, 1, box=4.0)
+    pb = ParamBasis.random_h_cluster(4, 1, box=4.0)
+    Er, gr, _ = rhf_gradient(nuc, pb, TIGHT, schwarz=0.0)
+    Eu, gu, res = uhf_gradient(nuc, pb, TIGHT_U, schwarz=0.0)
+    assert res.converged
+    _say("closed shell: E_RHF %.12f E_UHF %.12f, max |g_UHF - g_RHF| %.2e" % (Er, Eu, np.abs(gu - gr).max()))
+    assert Eu == pytest.approx(Er, abs=1e-9)
+    # the two SCFs stop at 1e-12 Eh from different iterations, so their
+    # densities (and the gradients, which are linear in them away from
+    # self-consistency) agree to ~1e-8, not to rounding
+    np.testing.assert_allclose(gu, gr, atol=5e-8, rtol=1e-6)
+
+
+def test_uhf_open_shell_against_central_differences():
+    """A 5-atom H cluster (doublet): the UHF gradient against central
+    differences of converged GPU UHF energies (every third parameter)."""
+    nuc, _ = h_cluster_sp(5, 3, box=4.5)
+    pb = ParamBasis.random_h_cluster(5, 3, box=4.5)
+    E, g, res = uhf_gradient(nuc, pb, TIGHT_U, schwarz=0.0)
+    assert res.converged
+    _say("5-atom H cluster UHF, E %.10f, %d parameters" % (E, pb.n_theta))
+    idx = list(range(0, pb.n_theta, 3))
+    fd = np.array([_fd(nuc, pb, i, cfg=TIGHT_U) for i in idx])
+    err = np.abs(g[idx] - fd)
+    _say("  max |adjoint - FD| %.2e, max |g| %.2e" % (err.max(), np.abs(g).max()))
+    np.testing.assert_allclose(g[idx], fd, atol=2e-6, rtol=1e-5)
+
+
+def test_uhf_against_oracle_scf_differences():
+    nuc, _ = h_cluster_sp(5, 4, box=4.5)
+    pb = ParamBasis.random_h_cluster(5, 4, box=4.5)
+    E, g, _ = uhf_gradient(nuc, pb, TIGHT_U, schwarz=0.0)
+    assert E == pytest.approx(_energy(nuc, pb, "oracle", TIGHT_U), abs=1e-10)
+    rng = np.random.default_rng(1)
+    for i in rng.choice(pb.n_theta, size=6, replace=False):
+        fd = _fd(nuc, pb, int(i), "oracle", TIGHT_U)
+        _say("  UHF theta[%d]: adjoint %.10f oracle FD %.10f" % (i, g[i], fd))
+        assert g[i] == pytest.approx(fd, abs=2e-6, rel=1e-5)
