@@ -119,6 +119,22 @@ def traffic_from_profile(tag):
     return None
 
 
+def build_traffic_from_profile():
+    """Sum over the classes of the committed PMC summary's bytes per launch:
+    the L2-miss (HBM-side) bytes of one FP64 Fock build's J/K kernels, or None."""
+    path = os.path.join(ROOT, "profiles", "eri_jk64_hbm_traffic_592bf.txt")
+    if not os.path.exists(path):
+        return None
+    tot, seen = 0.0, False
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if len(w) >= 3 and w[0] == "traffic":
+                tot += float(w[2])
+                seen = True
+    return tot if seen else None
+
+
 def core_guess_density(gb, nuc, nocc):
     import quiqbox_amd as qb
     from quiqbox_amd.hartree_fock import getC, getD
@@ -664,6 +680,15 @@ def main():
                      "serialized_frac": None if args.timed_only else achieved_ser / PEAK_FP64_TFLOPS,
                      "build_tflops": build_tflops, "build_frac": build_tflops / PEAK_FP64_TFLOPS},
     }
+    # J/K bytes per second: the profile's bytes per launch over the measured times
+    tr, btr = traffic_from_profile(tag), build_traffic_from_profile()
+    out["roofline"]["kernel_hbm_gbs"] = tr / (1e-3 * float(conc[dom])) / 1e9 if tr else None
+    out["roofline"]["build_hbm_bytes"] = btr
+    out["roofline"]["build_hbm_gbs"] = btr / (1e-3 * ms_step) / 1e9 if btr else None
+    out["roofline"]["hbm_peak_gbs"] = 8000.0
+    if btr:
+        log("J/K traffic: %.1f GB of L2 misses per build (PMC summary) -> %.0f GB/s over the build (%.1f%% of the "
+            "8 TB/s HBM peak)" % (btr / 1e9, out["roofline"]["build_hbm_gbs"], out["roofline"]["build_hbm_gbs"] / 80))
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(gb.flat, n_cart, args.cpu_seconds)
