@@ -99,6 +99,13 @@ STAGE_SPLIT = {(2, 2, 2, 2): 4, (2, 2, 2, 1): 4, (2, 1, 2, 1): 4}
 # one); the concurrent build was 0.5-1 ms slower with one-chunk tasks for all
 # three than with two (more K-row flushes among the other classes' atomics).
 STAGE_SPLIT_CHUNKS = {(2, 2, 2, 2): 1, (2, 2, 2, 1): 2, (2, 1, 2, 1): 2}
+# Classes whose K-row J/K kernel is compiled for two waves per SIMD (256
+# registers, the rest spilled to scratch) instead of the compiler's choice of
+# one wave with AGPRs.  Measured on the 592-function cluster, serialized:
+# (ds|pp) 1.93 -> 1.75 ms, (dp|ps) 3.34 -> 3.30 ms, build 32.3-32.5 -> 32.1 ms.
+# The classes that need 400+ registers ((pp|pp), (dp|ds), (dd|ps)) would
+# spill 640-940 bytes per lane.
+ROWS_TWO_WAVES = {(2, 1, 1, 0), (2, 0, 1, 1)}
 TWIN_MAX_ACC = 9   # must match qbx_eri_api.h
 
 
@@ -713,6 +720,9 @@ def gen_class(la, lb, lc, ld):
     code.append("    static constexpr int NACC = %d;" % (nE * nF))
     code.append("    static constexpr int NE = %d;" % nE)
     code.append("    static constexpr int LT = %d;" % Lt)
+    code.append("    // FP64 K-row J/K kernel compiled for two waves per SIMD")
+    code.append("    static constexpr bool ROWS_TWO_WAVES = %s;" % (
+        "std::is_same<R, double>::value" if (la, lb, lc, ld) in ROWS_TWO_WAVES else "false"))
     flops_exec = None
     if (la, lb, lc, ld) in STAGED:
         scode, fve, fbra = gen_staged(la, lb, lc, ld, es, fs, eidx, fidx, C_, D_)

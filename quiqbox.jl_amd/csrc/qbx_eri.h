@@ -22,6 +22,7 @@
 #include "qbx_eri_api.h"
 
 #include <cstdint>
+#include <type_traits>
 #include <utility>
 
 template <int LA, int LB, int LC, int LD, typename R> struct EriClass;
@@ -605,7 +606,34 @@ __device__ __forceinline__ void digest_rows(const EriArgs& a, const R* kacc, con
 // J_cd goes out per lane, as in eri_kernel.
 // ---------------------------------------------------------------------------
 template <int LA, int LB, int LC, int LD, typename R>
+__device__ __forceinline__ void jk_rows_body(const EriArgs& a);
+
+template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(64 * ROW_WAVES) void eri_jk_rows_kernel(EriArgs a)
+{
+    jk_rows_body<LA, LB, LC, LD, R>(a);
+}
+
+// The same kernel held to two waves per SIMD (EriClass::ROWS_TWO_WAVES): 256
+// registers a wave, the rest spilled to scratch, where the compiler would
+// otherwise run one wave with AGPRs.
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ __launch_bounds__(64 * ROW_WAVES) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void eri_jk_rows2_kernel(EriArgs a)
+{
+    jk_rows_body<LA, LB, LC, LD, R>(a);
+}
+
+// the K-row kernel of a class (only that one is instantiated)
+template <int LA, int LB, int LC, int LD, typename R>
+constexpr auto rows_kernel()
+{
+    if constexpr (EriClass<LA, LB, LC, LD, R>::ROWS_TWO_WAVES) return &eri_jk_rows2_kernel<LA, LB, LC, LD, R>;
+    else return &eri_jk_rows_kernel<LA, LB, LC, LD, R>;
+}
+
+template <int LA, int LB, int LC, int LD, typename R>
+__device__ __forceinline__ void jk_rows_body(const EriArgs& a)
 {
     using Cls = EriClass<LA, LB, LC, LD, R>;
     constexpr int TPB = 64 * ROW_WAVES;
@@ -1448,7 +1476,7 @@ void launch_eri(const EriArgs& a, hipStream_t s)
             // the attribute call fails where the device cannot give a block
             // that much LDS; the one-wave-per-chunk kernel runs then
             static const bool rows_ok = [] {
-                const bool ok = hipFuncSetAttribute((const void*)eri_jk_rows_kernel<LA, LB, LC, LD, R>,
+                const bool ok = hipFuncSetAttribute((const void*)rows_kernel<LA, LB, LC, LD, R>(),
                                                     hipFuncAttributeMaxDynamicSharedMemorySize,
                                                     ROW_LDS_MAX) == hipSuccess;
                 (void)hipGetLastError();
@@ -1458,12 +1486,12 @@ void launch_eri(const EriArgs& a, hipStream_t s)
                 if (a.nsuper <= 0) return;
                 static const size_t scratch = [] {
                     hipFuncAttributes fa;
-                    if (hipFuncGetAttributes(&fa, (const void*)eri_jk_rows_kernel<LA, LB, LC, LD, R>) != hipSuccess)
+                    if (hipFuncGetAttributes(&fa, (const void*)rows_kernel<LA, LB, LC, LD, R>()) != hipSuccess)
                         return (size_t)1;
                     return (size_t)fa.localSizeBytes;
                 }();
                 hipStream_t use = (scratch > 0 && a.scratch_stream) ? (hipStream_t)a.scratch_stream : s;
-                hipLaunchKernelGGL((eri_jk_rows_kernel<LA, LB, LC, LD, R>), dim3((unsigned)a.nsuper),
+                hipLaunchKernelGGL((rows_kernel<LA, LB, LC, LD, R>()), dim3((unsigned)a.nsuper),
                                    dim3(64 * ROW_WAVES), shm, use, a);
                 QBX_HIP(hipGetLastError());
                 return;
