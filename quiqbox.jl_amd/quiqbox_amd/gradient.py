@@ -1,5 +1,5 @@
 """RHF and UHF energy gradients with respect to the basis parameters, and basis-set
-optimization built on it.
+optimization built on them.
 
 Quiqbox v0.6.3 keeps the parameters of a basis in boxes (`Parameters.jl:14-43`,
 `genCellParam` `:387-405`) that a caller can mark as variables, but it offers
@@ -28,6 +28,10 @@ fixed, and `rhf_gradient` chains it through the normalization above:
 
 with t_mu = 2 (D F - W S)_mumu, the derivative with respect to scaling
 function mu (zero at self-consistency, kept for finite convergence).
+
+For a UHF state (`uhf_gradient`, qbx_uhf_gradient) Gamma takes the exchange
+of each spin density, W sums both spins' C_occ diag(eps) C_occ^T, and
+t_mu = 2 sum_s (D_s F_s)_mumu - 2 (W S)_mumu.
 """
 import math
 
