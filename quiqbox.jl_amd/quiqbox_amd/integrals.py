@@ -5,14 +5,46 @@ The `lazyCompute` keyword is accepted for drop-in compatibility; the GPU path
 evaluates every requested integral directly (Quiqbox's lazy and eager paths
 are bit-identical by contract, Coulomb-test.jl:102,149).
 """
+import collections
+import hashlib
+import threading
+
 import numpy as np
 
+from .basis import flatten_basis
 from .engine import GPUBasis, default_context
 from .particles import NuclearCluster, getCharge
 
+# Small bases (the single-element calls: two or four functions) are kept on
+# the GPU while their flattened content is unchanged, so repeated calls on the
+# same functions build them once; the oldest is dropped beyond _CACHE_MAX.
+_CACHE_MAX_BF = 64
+_CACHE_MAX = 32
+_cache = collections.OrderedDict()
+_cache_lock = threading.Lock()
+
 
 def _basis(orbs):
-    return GPUBasis(list(orbs))
+    flat = flatten_basis(list(orbs))
+    if flat.n_bf > _CACHE_MAX_BF:
+        return GPUBasis(flat)
+    h = hashlib.sha1()
+    for a in (flat.prim_cen, flat.prim_xpn, flat.prim_ang, flat.bf_ptr, flat.bf_prim, flat.bf_w):
+        a = np.ascontiguousarray(a)
+        h.update(str(a.dtype).encode() + str(a.shape).encode())
+        h.update(a.tobytes())
+    key = h.digest()
+    with _cache_lock:
+        gb = _cache.get(key)
+        if gb is not None:
+            _cache.move_to_end(key)
+            return gb
+    gb = GPUBasis(flat)
+    with _cache_lock:
+        _cache[key] = gb
+        while len(_cache) > _CACHE_MAX:
+            _cache.popitem(last=False)
+    return gb
 
 
 def _nuclei(nucs, coords):
