@@ -380,10 +380,10 @@ class LearnedGuess:
 
 
 # The C/H/O network shipped with the package: trained by
-# `python -m quiqbox_amd.train_guess --systems 1600 --min-atoms 8 --max-atoms 16
+# `python -m quiqbox_amd.train_guess --systems 4000 --min-atoms 8 --max-atoms 16
 # --epochs 150 --batch 2048` on the GPU-converged RHF densities of random C/H/O
-# clusters (cho_training_systems, seeds 100000-101599: 801 of the 1600
-# converged, 110,726 atom pairs); the file's `meta` holds the counts and times.
+# clusters (cho_training_systems, seeds 100000-103999: 2016 of the 4000
+# converged, 285,425 atom pairs); the file's `meta` holds the counts and times.
 CHO_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "cho_sp_guess.npz")
 
 

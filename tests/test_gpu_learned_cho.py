@@ -66,15 +66,16 @@ def test_cho_learned_guess_on_unseen_clusters():
 
 def test_packaged_cho_network_on_unseen_clusters():
     """The C/H/O network shipped with the package (quiqbox_amd/data/cho_sp_guess.npz,
-    trained by quiqbox_amd.train_guess on 1600 generated 8-16-atom clusters) loads
+    trained by quiqbox_amd.train_guess on 4000 generated 8-16-atom clusters) loads
     with its training record and gives a finite guess within 30% of the electron
     count on every unseen cluster; the SCF outcomes next to the core guess are
     printed (they are chaotic, as above)."""
     model = lg.load_cho_model()
     _say("packaged network: %s" % model.meta)
     assert model.layout.elements == lg.CHO_SP.elements and model.layout.nmax == lg.CHO_SP.nmax
-    assert model.meta["systems"] == 1600 and model.meta["converged"] >= 400
-    assert model.meta["loss_last"] < 0.5 * model.meta["loss_first"]
+    assert model.meta["systems"] == 4000 and model.meta["converged"] >= 1000
+    # the loss after the first epoch is already 0.18 over 285k pairs; it levels off at 0.11
+    assert model.meta["loss_last"] < 0.8 * model.meta["loss_first"] and model.meta["loss_last"] < 0.15
     strat = qb.SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-8))
     for n, seed in HELD_OUT:
         nuc, bs = lg.cho_cluster(n, seed)

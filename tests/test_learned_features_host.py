@@ -80,4 +80,4 @@ def test_packaged_cho_network_file():
     assert z["mu"].shape == (L.n_feat,) and z["sd"].shape == (L.n_feat,)
     assert all(np.all(np.isfinite(z[k])) for k in ("W0", "W1", "W2", "b0", "b1", "b2", "mu", "sd"))
     meta = json.loads(str(z["meta"]))
-    assert meta["systems"] == 1600 and meta["atoms"] == [8, 16] and meta["pairs"] > 0
+    assert meta["systems"] == 4000 and meta["atoms"] == [8, 16] and meta["pairs"] > 0
