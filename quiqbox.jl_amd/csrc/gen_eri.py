@@ -106,6 +106,15 @@ STAGE_SPLIT_CHUNKS = {(2, 2, 2, 2): 1, (2, 2, 2, 1): 2, (2, 1, 2, 1): 2}
 # The classes that need 400+ registers ((pp|pp), (dp|ds), (dd|ps)) would
 # spill 640-940 bytes per lane.
 ROWS_TWO_WAVES = {(2, 1, 1, 0), (2, 0, 1, 1)}
+# Classes launched after all others in a Fock build.  (dp|ds)'s K-row kernel
+# needs 512 registers a wave (one wave per SIMD, a 4-wave block per CU) and
+# spills to scratch, so it runs on the main stream, and its blocks start only
+# on CUs that have drained completely.  Queued behind (dp|ps) and (ds|pp) it
+# starts when the staged classes are done and holds CUs for a shorter stretch:
+# its average duration in the concurrent build 18.2 -> 14.2 ms, the build
+# 32.30 -> 32.42 ms (five A/B pairs on the 592-function cluster, within the
+# run-to-run spread).
+LAUNCH_LAST = {(2, 1, 2, 0)}
 TWIN_MAX_ACC = 9   # must match qbx_eri_api.h
 
 
@@ -818,6 +827,9 @@ def main(outdir):
              "static const int qbx_class_split_chunks[%d] = {%s};" % (
                  len(classes), ", ".join(str(STAGE_SPLIT_CHUNKS[c] if c in STAGED and STAGE_SPLIT.get(c, 1) > 1
                                              else 0) for c in classes)),
+             "// classes launched after all others in a Fock build",
+             "static const int qbx_class_last[%d] = {%s};" % (
+                 len(classes), ", ".join("1" if c in LAUNCH_LAST else "0" for c in classes)),
              "// classes with a twin-bra kernel (not staged, at most %d accumulators)" % TWIN_MAX_ACC,
              "static const int qbx_class_twin[%d] = {%s};" % (
                  len(classes), ", ".join("1" if t else "0" for t in twin_ok)),

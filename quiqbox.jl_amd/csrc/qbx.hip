@@ -1644,7 +1644,9 @@ void jk_device(qbx_ctx* ctx, qbx_basis* b, int nset, const double* dD, double* d
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         // staged classes first: few long-running waves each, they overlap with
         // the wide low-L kernels instead of running alone at the end; then the
-        // largest estimated work first, so the short classes fill the tail
+        // largest estimated work first, so the short classes fill the tail;
+        // the classes of qbx_class_last after all of them
+        if (qbx_class_last[x] != qbx_class_last[y]) return qbx_class_last[y] != 0;
         const bool sx = qbx_class_ws[x] > 0, sy = qbx_class_ws[y] > 0;
         if (sx != sy) return sx;
         return b->class_cost[x] > b->class_cost[y];
