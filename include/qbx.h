@@ -233,7 +233,7 @@ int  qbx_rhf_gradient(qbx_ctx* ctx, qbx_basis* basis, const double* Dt, const do
                       double* g_cen);
 
 /* The same gradient for a UHF state (the reference's UOHartreeFock,
-   HartreeFock.jl:17-24): Dt = D_alpha + D_beta, the spin densities
+   HartreeFock.jl:63; its G, :322-327): Dt = D_alpha + D_beta, the spin densities
    D_s = C_s,occ C_s,occ^T, and W = sum_s C_s,occ diag(eps_s,occ) C_s,occ^T;
    the two-electron term is sum (1/2 Dt_mn Dt_ls - 1/2 sum_s D_s,ml D_s,ns)
    (mn|ls).  With D_alpha = D_beta = Dt/2 it equals qbx_rhf_gradient. */
