@@ -233,8 +233,19 @@ class GPUBasis:
             M = self.T.T @ M @ self.T
         return M
 
+    def _cached(self, name, key, make):
+        """A copy of the one-electron matrix `name` for `key`, computed once:
+        the basis is fixed, so S, T and the core Hamiltonian of the last
+        nuclei asked for are kept (one entry per matrix)."""
+        c = self.__dict__.setdefault("_one_e", {})
+        hit = c.get(name)
+        if hit is None or hit[0] != key:
+            hit = (key, make())
+            c[name] = hit
+        return hit[1].copy()
+
     def overlap(self):
-        return self._mat_out(_lib.load().qbx_overlap)
+        return self._cached("S", None, lambda: self._mat_out(_lib.load().qbx_overlap))
 
     def orthonormalization(self):
         """X = S^{-1/2} of this basis (HartreeFock.jl:39-42), computed once per
@@ -247,7 +258,7 @@ class GPUBasis:
         return self._X
 
     def kinetic(self):
-        return self._mat_out(_lib.load().qbx_kinetic)
+        return self._cached("T", None, lambda: self._mat_out(_lib.load().qbx_kinetic))
 
     def nuclear(self, Z, R):
         Z = _lib.f64(Z)
@@ -267,7 +278,8 @@ class GPUBasis:
     def core_hamiltonian(self, Z, R):
         Z = _lib.f64(Z)
         R = _lib.f64(R).reshape(-1)
-        return self._mat_out(_lib.load().qbx_core_hamiltonian, len(Z), _lib.dp(Z), _lib.dp(R))
+        return self._cached("H", (Z.tobytes(), R.tobytes()),
+                            lambda: self._mat_out(_lib.load().qbx_core_hamiltonian, len(Z), _lib.dp(Z), _lib.dp(R)))
 
     def multipole(self, center, degree):
         c = _lib.f64(center)
