@@ -680,6 +680,10 @@ def main():
                      "serialized_frac": None if args.timed_only else achieved_ser / PEAK_FP64_TFLOPS,
                      "build_tflops": build_tflops, "build_frac": build_tflops / PEAK_FP64_TFLOPS},
     }
+    from quiqbox_amd import _lib
+    prov = _lib.provenance()
+    out["library"] = {"source_digest": prov["library"], "tree_source_digest": prov["sources"],
+                      "built_from_these_sources": prov["match"]}
     # J/K bytes per second: the profile's bytes per launch over the measured times
     tr, btr = traffic_from_profile(tag), build_traffic_from_profile()
     out["roofline"]["kernel_hbm_gbs"] = tr / (1e-3 * float(conc[dom])) / 1e9 if tr else None

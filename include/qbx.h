@@ -319,6 +319,12 @@ int  qbx_mlp_adam(int64_t n, float* w, float* m, float* v, const float* g, void*
 /* Last error message of the calling thread; returns its full length. */
 int  qbx_last_error(char* buf, size_t len);
 
+/* Digest (16 hex digits of a sha256) of the sources this library was built
+   from, as quiqbox.jl_amd/build.py computes it ("unknown" for a build without
+   it).  No reference counterpart: build provenance, so that a run can tell a
+   library built from the sources beside it from a stale one. */
+const char* qbx_source_digest(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,23 @@ FLAGS = [
 ]
 
 
+def source_digest():
+    """sha256 (first 16 hex digits) over the sources the library is built
+    from: csrc/*.h, csrc/*.hip (not the generated ones, which follow from
+    gen_eri.py), csrc/gen_eri.py, include/qbx.h and this script.  The library
+    reports the digest it was built from (qbx_source_digest), so a run can
+    tell whether it loaded a library built from the sources beside it."""
+    paths = sorted(glob.glob(os.path.join(CSRC, "*.h")) + glob.glob(os.path.join(CSRC, "*.hip")) +
+                   [os.path.join(CSRC, "gen_eri.py"), os.path.join(INCLUDE, "qbx.h"), os.path.abspath(__file__)])
+    h = hashlib.sha256()
+    for p in paths:
+        h.update(os.path.relpath(p, os.path.dirname(PKG_ROOT)).encode() + b"\0")
+        with open(p, "rb") as f:
+            h.update(f.read())
+        h.update(b"\0")
+    return h.hexdigest()[:16]
+
+
 def generate():
     for old in glob.glob(os.path.join(GEN, "eri_*.hip")):
         os.remove(old)
@@ -80,8 +97,10 @@ def build(verbose=True, extra=(), jobs=None, force=False):
                                                      key=lambda p: -os.path.getsize(p))
     jobs = jobs or max(1, min(len(srcs), os.cpu_count() or 4))
     objs = {}
+    digest_flag = ['-DQBX_SOURCE_DIGEST="%s"' % source_digest()]
     with cf.ThreadPoolExecutor(jobs) as ex:
-        futs = {ex.submit(_compile, s, extra, headers, verbose, force): s for s in srcs}
+        futs = {ex.submit(_compile, s, list(extra) + (digest_flag if s.endswith("qbx.hip") else []), headers,
+                          verbose, force): s for s in srcs}
         for fu in cf.as_completed(futs):
             obj, dt, msg = fu.result()
             objs[futs[fu]] = obj

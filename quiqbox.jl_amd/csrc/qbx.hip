@@ -2815,6 +2815,15 @@ int qbx_uhf_gradient(qbx_ctx* ctx, qbx_basis* b, const double* Dt, const double*
     });
 }
 
+const char* qbx_source_digest(void)
+{
+#ifdef QBX_SOURCE_DIGEST
+    return QBX_SOURCE_DIGEST;
+#else
+    return "unknown";
+#endif
+}
+
 int qbx_set_mixed_cut(qbx_ctx* ctx, double cut)
 {
     return guarded([&] {

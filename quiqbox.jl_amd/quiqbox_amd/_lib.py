@@ -56,6 +56,7 @@ SIGNATURES = {
     "qbx_basis_shells": (_i32, [_vp, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd]),
     "qbx_rhf_gradient": (_i32, [_vp, _vp, _pd, _pd, _i32, _pd, _pd, _d, _pd, _pd, _pd]),
     "qbx_uhf_gradient": (_i32, [_vp, _vp, _pd, _pd, _pd, _pd, _i32, _pd, _pd, _d, _pd, _pd, _pd]),
+    "qbx_source_digest": (ctypes.c_char_p, []),
     "qbx_density_purify": (_i32, [_vp, _i32, _i32, _pd, _pd, _pd, _i32, _pi, _pd]),
     "qbx_fock_solve": (_i32, [_vp, _i32, _i32, _pd, _pd, _pd, _pd, _pd]),
     "qbx_density_purify_noalloc": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp, _i32, _pi, _pd, _vp]),
@@ -103,6 +104,22 @@ def load(path=LIB_PATH):
         fn.argtypes = args
     _lib = L
     return L
+
+
+def provenance():
+    """The library's source digest (qbx_source_digest), the digest of the
+    sources beside it (build.source_digest, None where they are absent) and
+    whether the two agree."""
+    lib = load().qbx_source_digest().decode()
+    src = None
+    build_py = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "build.py")
+    if os.path.exists(build_py):
+        import importlib.util
+        spec = importlib.util.spec_from_file_location("qbx_build_script", build_py)
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        src = mod.source_digest()
+    return dict(library=lib, sources=src, match=(src is not None and lib == src))
 
 
 def last_error():

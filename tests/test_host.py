@@ -109,3 +109,14 @@ def test_workload_generators():
     assert r.min() >= 1.8
     nuc1, bs1 = h10_sto3g()
     assert len(bs1) == 10
+
+
+def test_library_built_from_these_sources():
+    """libqbx_hip.so reports the digest of the sources it was built from
+    (qbx_source_digest); it must be the digest of the sources in this tree,
+    i.e. the library was rebuilt after the last source change."""
+    from quiqbox_amd import _lib
+    prov = _lib.provenance()
+    assert prov["sources"] is not None and len(prov["library"]) == 16
+    assert prov["match"], "libqbx_hip.so was built from other sources (%s, tree %s): run build()" % (
+        prov["library"], prov["sources"])
