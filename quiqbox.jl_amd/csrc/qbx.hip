@@ -2140,27 +2140,34 @@ int qbx_eri_dense(qbx_ctx* ctx, const qbx_basis* bc, double* eri)
         DevBuf<double> d;
         d.ensure(N4);
         zero_async(d.p, N4, st);
-        for (int cls = 0; cls < QBX_N_CLASSES; ++cls) {
+        // each class's task lists (host and device) stay alive until the one
+        // synchronization below, so the classes' kernels queue back to back
+        struct DenseTasks {
             TaskCounts tc;
+            DevBuf<int64_t> pre;
+            DevBuf<int> ko, cw;
+        };
+        std::vector<std::unique_ptr<DenseTasks>> lists;
+        for (int cls = 0; cls < QBX_N_CLASSES; ++cls) {
+            auto L = std::make_unique<DenseTasks>();
+            TaskCounts& tc = L->tc;
             task_counts(b, cls, 0.0, QBX_PREC_FP64, 0.0, tc);
             const int nt = tc.ntb;
             if (tc.p64[nt] == 0) continue;
-            DevBuf<int64_t> pre;
-            DevBuf<int> ko, cw;
-            pre.upload(tc.p64, st);
-            ko.upload(tc.k64, st);
-            cw.upload(tc.w64, st);
+            L->pre.upload(tc.p64, st);
+            L->ko.upload(tc.k64, st);
+            L->cw.upload(tc.w64, st);
             EriArgs a = base_args(b, cls);
-            a.prefix = pre.p;
-            a.koff = ko.p;
-            a.cpre = cw.p;
+            a.prefix = L->pre.p;
+            a.koff = L->ko.p;
+            a.cpre = L->cw.p;
             a.total = tc.p64[nt];
             a.nwave = tc.w64[nt];
             a.nbra = nt;
             attach_workspace(b, cls, a.nwave, a);
             a.eri = d.p;
             k_dense[cls](a, st);
-            QBX_HIP(hipStreamSynchronize(st));
+            lists.push_back(std::move(L));
         }
         if (b->has_high) {
             ensure_hl_tasks(b, 0.0);
