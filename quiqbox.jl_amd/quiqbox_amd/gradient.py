@@ -38,7 +38,7 @@ import math
 import numpy as np
 
 from . import systems
-from .basis import PrimitiveOrb, get3DimPGTOrbNormFactor, prim_overlap, subshellXYZs
+from .basis import subshellXYZs
 from .engine import GPUBasis
 from .hartree_fock import HFconfig, OrbCoeffInitialConfig, RCHartreeFock, SCFconfig, runHartreeFock
 
@@ -102,28 +102,28 @@ class ParamBasis:
         return bs
 
 
-def _lnS_derivatives(L, ijk, x, c, h=1e-6):
-    """d ln S_mumu / d c_k and d ln S_mumu / d alpha_k of one function before
-    renormalization, S = sum_kl c_k c_l N_k N_l <g_k|g_l> (same centre)."""
-    def lnS(xv, cv):
-        S = 0.0
-        for a, ca in zip(xv, cv):
-            for b, cb in zip(xv, cv):
-                pa = PrimitiveOrb((0.0, 0.0, 0.0), a, ijk)
-                pb = PrimitiveOrb((0.0, 0.0, 0.0), b, ijk)
-                S += ca * cb * get3DimPGTOrbNormFactor(a, ijk) * get3DimPGTOrbNormFactor(b, ijk) * \
-                    prim_overlap(pa, pb)
-        return math.log(S)
-    dc, dx = np.zeros(len(x)), np.zeros(len(x))
-    for k in range(len(x)):
-        cp, cm = c.copy(), c.copy()
-        cp[k] += h * c[k]
-        cm[k] -= h * c[k]
-        dc[k] = (lnS(x, cp) - lnS(x, cm)) / (2 * h * c[k])
-        xp, xm = x.copy(), x.copy()
-        xp[k] += h * x[k]
-        xm[k] -= h * x[k]
-        dx[k] = (lnS(xp, c) - lnS(xm, c)) / (2 * h * x[k])
+def _lnS_derivatives(L, x, c):
+    """d ln S_mumu / d c_k and d ln S_mumu / d alpha_k of one function of a
+    shell of angular momentum L before renormalization,
+    S = sum_kl c_k c_l N_k N_l <g_k|g_l> (same centre).  The primitive norm
+    N(alpha, ijk) is alpha^((2L+3)/4) times a factor of ijk alone, and
+    <g_k|g_l> is (alpha_k + alpha_l)^-(L+3/2) times a factor of ijk alone, so
+    with f_k = c_k alpha_k^a (a = 3/4 + L/2) and O_kl = (alpha_k + alpha_l)^-b
+    (b = L + 3/2), ln S = const + ln(f' O f) for every Cartesian component:
+        d ln S/dc_k     = 2 alpha_k^a (O f)_k / S
+        d ln S/dalpha_k = 2 f_k (a/alpha_k (O f)_k - b (O'' f)_k) / S,
+    O''_kl = O_kl / (alpha_k + alpha_l)."""
+    x = np.asarray(x, dtype=np.float64)
+    c = np.asarray(c, dtype=np.float64)
+    a, b = 0.75 + 0.5 * L, L + 1.5
+    xa = x ** a
+    f = c * xa
+    P = x[:, None] + x[None, :]
+    O = P ** (-b)
+    Of = O @ f
+    S = float(f @ Of)
+    dc = 2.0 * xa * Of / S
+    dx = 2.0 * f * (a / x * Of - b * ((O / P) @ f)) / S
     return dc, dx
 
 
@@ -149,12 +149,12 @@ def _chain(pb, gx, gl, gc, sh, t):
         gxs, gls = gx[o:o + len(x)], gl[o:o + len(x)]
         dEdc = gls / c
         dEdx = gxs + (0.75 + 0.5 * L) * gls / x
-        for i, ijk in enumerate(subshellXYZs(L)):
-            if abs(t[f0 + i]) < 1e-14:      # self-consistent: the scaling derivative vanishes
-                continue
-            dc, dx = _lnS_derivatives(L, ijk, x, c)
-            dEdc = dEdc - 0.5 * t[f0 + i] * dc
-            dEdx = dEdx - 0.5 * t[f0 + i] * dx
+        # every component of the shell has the same d ln S (see _lnS_derivatives)
+        tsum = float(np.sum(t[f0:f0 + len(subshellXYZs(L))]))
+        if tsum != 0.0:
+            dc, dx = _lnS_derivatives(L, x, c)
+            dEdc = dEdc - 0.5 * tsum * dc
+            dEdx = dEdx - 0.5 * tsum * dx
         g_x.append(dEdx)
         g_c.append(dEdc)
         gR[s] += gc[k]
