@@ -292,6 +292,14 @@ class PurifiedDensity:
         return 0.5 * (D + D.T)
 
 
+# Basis size from which the default SCF step runs on the GPU.  One step
+# (X' F X, eigen, C = X C', D) measured on the box, device against numpy on
+# the host: n = 128 2.44 / 1.18 ms, 256 6.18 / 4.38 ms, 384 8.86 / 8.79 ms,
+# 512 12.0 / 23.0 ms, 600 13.6 / 18.0 ms (rocSOLVER's fixed costs dominate
+# small problems).
+DEVICE_EIGEN_MIN_N = 384
+
+
 class DeviceEigen:
     """getC + getD of the SCF steps on the GPU (Context.fock_solve): X' F X
     and X C' on the FP64 matrix cores, the symmetric eigenproblem by rocSOLVER;
@@ -1045,13 +1053,14 @@ def runHartreeFock(nucInfo, basis, config=None, *, printInfo=True, infoLevel=def
     `incremental` (density-screened incremental Fock builds) select the GPU
     Fock-build mode; `densitySolver` selects each step's orbitals and density:
     "eigen" (default) diagonalizes X' F X as the reference does, on the GPU
-    when the Fock build is (matrix-core GEMMs, rocSOLVER eigensolver),
-    "eigen-host" the same on the host (numpy), "purify" forms the density by
-    purification on the GPU instead (the same matrix; the coefficients of the
-    result still come from one diagonalization).  The defaults reproduce the
-    reference's full FP64 builds and eigen steps."""
-    if densitySolver not in ("eigen", "eigen-host", "purify"):
-        raise ValueError("densitySolver must be 'eigen', 'eigen-host' or 'purify'")
+    when the Fock build is and the basis has at least DEVICE_EIGEN_MIN_N
+    functions (matrix-core GEMMs, rocSOLVER eigensolver), on the host (numpy)
+    below that; "eigen-device" and "eigen-host" force either; "purify" forms
+    the density by purification on the GPU instead (the same matrix; the
+    coefficients of the result still come from one diagonalization).  The
+    defaults reproduce the reference's full FP64 builds and eigen steps."""
+    if densitySolver not in ("eigen", "eigen-device", "eigen-host", "purify"):
+        raise ValueError("densitySolver must be 'eigen', 'eigen-device', 'eigen-host' or 'purify'")
     t0 = time.time()
     if isinstance(nucInfo, tuple) and len(nucInfo) == 2 and isinstance(nucInfo[1], NuclearCluster):
         spin, nuc = nucInfo
@@ -1084,8 +1093,9 @@ def runHartreeFock(nucInfo, basis, config=None, *, printInfo=True, infoLevel=def
         if not isinstance(hamil.twoBody, DirectJK):
             raise ValueError("densitySolver='purify' runs with the GPU Fock build")
         solver = hamil.twoBody.density_solver = PurifiedDensity(gb.ctx)
-    elif (densitySolver == "eigen" and isinstance(hamil.twoBody, DirectJK) and
-          hasattr(getattr(gb, "ctx", None), "fock_solve")):
+    elif (densitySolver in ("eigen", "eigen-device") and isinstance(hamil.twoBody, DirectJK) and
+          hasattr(getattr(gb, "ctx", None), "fock_solve") and
+          (densitySolver == "eigen-device" or nb >= DEVICE_EIGEN_MIN_N)):
         eig = hamil.twoBody.eigen_solver = DeviceEigen(gb.ctx)
     t1 = time.time()
     if printInfo and infoLevel > 1:

@@ -171,7 +171,8 @@ def test_fock_solve_matches_host_eigen_step(system):
 
 
 def test_scf_default_device_eigen_matches_host_eigen():
-    """runHartreeFock's default step on the GPU (densitySolver="eigen") against
+    """runHartreeFock's step on the GPU (densitySolver="eigen-device", the default from
+    DEVICE_EIGEN_MIN_N functions on) against
     the same SCF with the host eigen step: same energy to 1e-10 Eh, same
     number of Fock builds (the trajectory only differs by rounding)."""
     L = literals()["h2o2_631g"]
@@ -179,7 +180,7 @@ def test_scf_default_device_eigen_matches_host_eigen():
     nuc = qb.NuclearCluster(L["nucs"], coords)
     gb = qb.GPUBasis(molecule_basis(nuc.symbols, nuc.coords, "6-31G"))
     cfg = qb.HFconfig(initial="CoreH", strategy=qb.SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-10)), maxStep=200)
-    dev = qb.runHartreeFock(nuc, gb, cfg, printInfo=False)
+    dev = qb.runHartreeFock(nuc, gb, cfg, printInfo=False, densitySolver="eigen-device")
     host = qb.runHartreeFock(nuc, gb, cfg, printInfo=False, densitySolver="eigen-host")
     _say("H2O2 SCF: device eigen E %.12f (%d device steps), host eigen E %.12f" % (
         dev.energy[0], dev.timing.get("device_eigen_steps", 0), host.energy[0]))
