@@ -226,17 +226,23 @@ def optimize_basis(nuc, pb, steps=100, lr=0.05, schwarz=1e-12, config=None, call
     """Plain gradient descent on theta in log space for exponents and
     coefficients (positive parameters) and in bohr for the centres:
     log x <- log x - lr * x dE/dx, R <- R - lr dE/dR.  Returns the final
-    ParamBasis and the energy history.  The SCF of each step starts from the
-    previous step's converged coefficients."""
+    ParamBasis and the energy history.  The SCF of each step after the first
+    starts from the previous step's converged coefficients, with DIIS alone to
+    the configuration's final threshold (the starting point is already near
+    convergence, where ADIIS/EDIIS only add their host-side solves: 100 steps
+    of the 32-atom cluster 23.1 -> 22.5 s)."""
     hist = []
     res = None
     nx = sum(len(x) for _, _, x, _ in pb.shells)
     base = config or HFconfig(initial="CoreH", strategy=SCFconfig(("ADIIS", "DIIS"), (1e-4, 1e-10)), maxStep=200)
+    st = base.strategy
+    warm = SCFconfig(("DIIS",), (st.interval[-1],), secondaryConvRatio=st.secondaryConvRatio,
+                     oscillateThreshold=st.oscillateThreshold)
     for it in range(steps):
         gb = GPUBasis(pb.orbitals())
         cfg = base
         if res is not None and res.converged:
-            cfg = HFconfig(initial=OrbCoeffInitialConfig(RCHartreeFock(), (res.coeff[0],)), strategy=base.strategy,
+            cfg = HFconfig(initial=OrbCoeffInitialConfig(RCHartreeFock(), (res.coeff[0],)), strategy=warm,
                            maxStep=base.maxStep)
         E, g, res = rhf_gradient(nuc, pb, cfg, schwarz, gpu_basis=gb)
         hist.append(E)
